@@ -1,0 +1,141 @@
+// Shared device/host helpers for liblbdem_hip.so (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "../../include/lbdem_hip.h"
+
+namespace lbdem {
+
+// ---------------------------------------------------------------------------------------
+// D3Q19 velocity set, in the reference's direction order (src/lbm3d/lbm_solver3d.py:60-91):
+// 0 rest; 1..6 faces (+x,-x,+y,-y,+z,-z); 7..18 edges.
+// ---------------------------------------------------------------------------------------
+constexpr int Q = 19;
+constexpr int kCX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int kCY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int kCZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int kOPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+constexpr double kW[Q] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                          1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+// ---------------------------------------------------------------------------------------
+// error reporting (thread-local message, see lbdem_last_error)
+// ---------------------------------------------------------------------------------------
+void set_error(const char *fmt, ...);
+
+#define LBDEM_CHECK_LAUNCH(name)                                              \
+  do {                                                                        \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess) {                                                   \
+      ::lbdem::set_error("%s: launch failed: %s", name, hipGetErrorString(e_)); \
+      return LBDEM_E_HIP;                                                     \
+    }                                                                         \
+  } while (0)
+
+#define LBDEM_REQUIRE(cond, ...)        \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::lbdem::set_error(__VA_ARGS__);  \
+      return LBDEM_E_ARG;               \
+    }                                   \
+  } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------------------
+// storage types: load/store with conversion to the compute type
+// ---------------------------------------------------------------------------------------
+template <typename S> struct Store;
+template <> struct Store<float> {
+  using T = float;
+  static __device__ __forceinline__ float ld(const float *p) { return __builtin_nontemporal_load(p); }
+  static __device__ __forceinline__ void st(float *p, float v) { __builtin_nontemporal_store(v, p); }
+};
+template <> struct Store<double> {
+  using T = double;
+  static __device__ __forceinline__ double ld(const double *p) { return __builtin_nontemporal_load(p); }
+  static __device__ __forceinline__ void st(double *p, double v) { __builtin_nontemporal_store(v, p); }
+};
+template <> struct Store<_Float16> {
+  using T = float;
+  static __device__ __forceinline__ float ld(const _Float16 *p) {
+    return (float)__builtin_nontemporal_load(p);
+  }
+  static __device__ __forceinline__ void st(_Float16 *p, float v) {
+    __builtin_nontemporal_store((_Float16)v, p);
+  }
+};
+template <> struct Store<__bf16> {
+  using T = float;
+  static __device__ __forceinline__ float ld(const __bf16 *p) {
+    const uint16_t b = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(p));
+    return __uint_as_float(((uint32_t)b) << 16);
+  }
+  static __device__ __forceinline__ void st(__bf16 *p, float v) {
+    const __bf16 h = (__bf16)v;  // round to nearest even (v_cvt_pk_bf16_f32)
+    __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h), reinterpret_cast<uint16_t *>(p));
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// kernel parameter block (passed by value)
+// ---------------------------------------------------------------------------------------
+struct KParams {
+  int nx, ny, nz;
+  int ncell;
+  void *f;
+  const uint8_t *flags;
+  const float *vel;
+  float *lag;
+  const uint32_t *psc_key;
+  const float *psc_grain;
+  float *psc_force;
+  const float *volfrac;
+  const float *velsolid;
+  const float *hforce;
+  const float *volfrac_prev;
+  const float *hforce_prev;
+  // physics constants
+  double omega0, tau0, cs2_smag, rho0, nu_lu;
+  double fx, fy, fz;
+  int has_force;
+  int t_odd;  // layout parity of the state being read (t % 2)
+  // outputs for macros / readout kernels
+  float *rho_out;
+  float *vel_out;
+  const float *mask;
+  void *out;
+  int out_f64;
+  const void *in;
+  double *work;
+};
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? n - 1 : (v >= n ? 0 : v); }
+
+// periodic neighbour x + (dx,dy,dz), as compute_neigh_index (src/lbm3d/lbm_solver3d.py:308-342)
+__device__ __forceinline__ int nbr_index(const KParams &p, int i, int j, int k, int dx, int dy, int dz) {
+  const int ii = wrapi(i + dx, p.nx), jj = wrapi(j + dy, p.ny), kk = wrapi(k + dz, p.nz);
+  return (ii * p.ny + jj) * p.nz + kk;
+}
+
+__device__ __forceinline__ void cell_ijk(const KParams &p, int c, int &i, int &j, int &k) {
+  const int nyz = p.ny * p.nz;
+  i = c / nyz;
+  const int r = c - i * nyz;
+  j = r / p.nz;
+  k = r - j * p.nz;
+}
+
+// ---------------------------------------------------------------------------------------
+// wave64 / block reductions in double
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace lbdem

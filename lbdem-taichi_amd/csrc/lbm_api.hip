@@ -1,0 +1,216 @@
+// extern "C" entry points of the lattice core (see include/lbdem_hip.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string>
+#include "common.hpp"
+#include "lbm_table.hpp"
+
+namespace lbdem {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+static const LbmKernels *table_for(int storage) {
+  switch (storage) {
+    case LBDEM_STORE_F32: return &lbm_kernels_f32();
+    case LBDEM_STORE_F16: return &lbm_kernels_f16();
+    case LBDEM_STORE_BF16: return &lbm_kernels_bf16();
+    case LBDEM_STORE_F64: return &lbm_kernels_f64();
+    default: return nullptr;
+  }
+}
+
+static int fill_params(const lbdem_lattice *lat, const lbdem_step_args *a, KParams &p, const char *who) {
+  LBDEM_REQUIRE(lat && a, "%s: null lattice or args", who);
+  LBDEM_REQUIRE(lat->nx > 0 && lat->ny > 0 && lat->nz > 0, "%s: bad extents %d x %d x %d", who, lat->nx,
+                lat->ny, lat->nz);
+  const int64_t n = (int64_t)lat->nx * lat->ny * lat->nz;
+  LBDEM_REQUIRE(n < (int64_t(1) << 31), "%s: %lld cells exceed 2^31", who, (long long)n);
+  LBDEM_REQUIRE(lat->coupling >= 0 && lat->coupling < 4, "%s: bad coupling %d", who, lat->coupling);
+  LBDEM_REQUIRE(lat->collision >= 0 && lat->collision < 3, "%s: bad collision %d", who, lat->collision);
+  LBDEM_REQUIRE(a->f && a->flags, "%s: f and flags are required", who);
+  LBDEM_REQUIRE(lat->omega0 > 0.0, "%s: omega0 must be positive", who);
+  p = KParams{};
+  p.nx = lat->nx;
+  p.ny = lat->ny;
+  p.nz = lat->nz;
+  p.ncell = (int)n;
+  p.f = a->f;
+  p.flags = a->flags;
+  p.vel = a->vel;
+  p.lag = a->lag;
+  p.psc_key = a->psc_key;
+  p.psc_grain = a->psc_grain;
+  p.psc_force = a->psc_force;
+  p.volfrac = a->volfrac;
+  p.velsolid = a->velsolid;
+  p.hforce = a->hforce;
+  p.volfrac_prev = a->volfrac_prev;
+  p.hforce_prev = a->hforce_prev;
+  p.omega0 = lat->omega0;
+  p.tau0 = 1.0 / lat->omega0;
+  p.cs2_smag = lat->cs_smag * lat->cs_smag;
+  p.rho0 = lat->rho0;
+  p.nu_lu = a->nu_lu;
+  p.fx = lat->force[0];
+  p.fy = lat->force[1];
+  p.fz = lat->force[2];
+  p.has_force = (lat->force[0] != 0.0 || lat->force[1] != 0.0 || lat->force[2] != 0.0) ? 1 : 0;
+  p.t_odd = (int)(a->t & 1);
+  return LBDEM_OK;
+}
+
+static int launch(KernelFn k, int ncell, const KParams &p, void *stream, const char *who) {
+  if (ncell == 0) return LBDEM_OK;
+  const unsigned blocks = (unsigned)((ncell + 255) / 256);
+  KParams pp = p;
+  void *args[] = {&pp};
+  hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(k), dim3(blocks), dim3(256), args, 0,
+                                 as_stream(stream));
+  if (e != hipSuccess) {
+    set_error("%s: launch failed: %s", who, hipGetErrorString(e));
+    return LBDEM_E_HIP;
+  }
+  return LBDEM_OK;
+}
+
+// deterministic fixed-order final sum of per-workgroup partials (one workgroup)
+__global__ __launch_bounds__(256) void k_reduce_final(const double *work, int nblocks, double *out) {
+  __shared__ double sh[256][5];
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) s[m] += work[(size_t)b * 5 + m];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) sh[threadIdx.x][m] = s[m];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) sh[threadIdx.x][m] += sh[threadIdx.x + w][m];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) out[threadIdx.x] = sh[0][threadIdx.x];
+}
+
+}  // namespace lbdem
+
+using namespace lbdem;
+
+extern "C" {
+
+int lbdem_abi_version(void) { return LBDEM_ABI_VERSION; }
+
+const char *lbdem_last_error(void) { return g_last_error.c_str(); }
+
+int lbdem_lbm_init(const lbdem_lattice *lat, const lbdem_step_args *a, const float *rho, void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_init");
+  if (rc) return rc;
+  LBDEM_REQUIRE(rho && a->vel, "lbdem_lbm_init: rho and vel are required");
+  LBDEM_REQUIRE(lat->collision != LBDEM_COLL_SMAG_LAGGED || a->lag,
+                "lbdem_lbm_init: lagged Smagorinsky needs the lag buffer");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_init: bad storage %d", lat->storage);
+  p.rho_out = const_cast<float *>(rho);
+  return launch(t->init[lat->coupling], p.ncell, p, stream, "lbdem_lbm_init");
+}
+
+int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_step");
+  if (rc) return rc;
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_step: bad storage %d", lat->storage);
+  int coll = lat->collision;
+  if (lat->coupling == LBDEM_COUPLE_VANS) coll = LBDEM_COLL_BGK;  // VANS carries its own omega
+  KernelFn k = t->step[coll][lat->coupling][p.t_odd];
+  if (!k) {
+    set_error("lbdem_lbm_step: collision %d with coupling %d is not supported", lat->collision, lat->coupling);
+    return LBDEM_E_UNSUPPORTED;
+  }
+  LBDEM_REQUIRE(coll != LBDEM_COLL_SMAG_LAGGED || a->lag, "lbdem_lbm_step: lagged Smagorinsky needs lag");
+  LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_PSC || (a->psc_key && a->psc_grain && a->psc_force),
+                "lbdem_lbm_step: PSC needs psc_key, psc_grain, psc_force");
+  LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_EQIBM || (a->volfrac && a->velsolid && a->nu_lu > 0),
+                "lbdem_lbm_step: EqIBM needs volfrac, velsolid and nu_lu > 0");
+  LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_VANS ||
+                    (a->volfrac && a->hforce && a->volfrac_prev && a->hforce_prev && a->nu_lu > 0),
+                "lbdem_lbm_step: VANS needs volfrac, hforce, volfrac_prev, hforce_prev, nu_lu > 0");
+  return launch(k, p.ncell, p, stream, "lbdem_lbm_step");
+}
+
+int lbdem_lbm_macros(const lbdem_lattice *lat, const lbdem_step_args *a, float *rho, float *vel,
+                     const float *mask, void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_macros");
+  if (rc) return rc;
+  LBDEM_REQUIRE(rho && vel, "lbdem_lbm_macros: rho and vel are required");
+  LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_VANS || (a->volfrac && a->hforce),
+                "lbdem_lbm_macros: VANS needs volfrac and hforce");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_macros: bad storage %d", lat->storage);
+  p.rho_out = rho;
+  p.vel_out = vel;
+  p.mask = mask;
+  return launch(t->macros[lat->coupling], p.ncell, p, stream, "lbdem_lbm_macros");
+}
+
+int lbdem_lbm_read_f(const lbdem_lattice *lat, const lbdem_step_args *a, void *out, int32_t out_f64,
+                     void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_read_f");
+  if (rc) return rc;
+  LBDEM_REQUIRE(out, "lbdem_lbm_read_f: out is required");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_read_f: bad storage %d", lat->storage);
+  p.out = out;
+  p.out_f64 = out_f64 ? 1 : 0;
+  return launch(t->read_f, p.ncell, p, stream, "lbdem_lbm_read_f");
+}
+
+int lbdem_lbm_write_f(const lbdem_lattice *lat, const lbdem_step_args *a, const void *in, int32_t in_f64,
+                      void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_write_f");
+  if (rc) return rc;
+  LBDEM_REQUIRE(in, "lbdem_lbm_write_f: in is required");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_write_f: bad storage %d", lat->storage);
+  p.in = in;
+  p.out_f64 = in_f64 ? 1 : 0;
+  return launch(t->write_f, p.ncell, p, stream, "lbdem_lbm_write_f");
+}
+
+int64_t lbdem_reduce_work_doubles(const lbdem_lattice *lat) {
+  if (!lat) return 0;
+  const int64_t n = (int64_t)lat->nx * lat->ny * lat->nz;
+  return ((n + 255) / 256) * 5;
+}
+
+int lbdem_lbm_reduce(const lbdem_lattice *lat, const lbdem_step_args *a, double *work, double *out,
+                     void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_reduce");
+  if (rc) return rc;
+  LBDEM_REQUIRE(work && out, "lbdem_lbm_reduce: work and out are required");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_reduce: bad storage %d", lat->storage);
+  p.work = work;
+  rc = launch(t->reduce_partial, p.ncell, p, stream, "lbdem_lbm_reduce");
+  if (rc) return rc;
+  const int nblocks = (p.ncell + 255) / 256;
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, as_stream(stream), work, nblocks, out);
+  LBDEM_CHECK_LAUNCH("lbdem_lbm_reduce");
+  return LBDEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,523 @@
+// Fused D3Q19 collide-stream kernels (AA pattern, SoA, shifted populations) for gfx950.
+//
+// One thread per cell, 256-thread workgroups (four wave64).  Even steps read and write the
+// cell's own 19 slots (writing to the opposite slot); odd steps gather from the neighbours'
+// opposite slots and push to the neighbours' slots.  Each slot is touched by exactly one
+// thread per step, so the update is in place and race-free: 19 loads + 19 stores per cell.
+// Cells flagged NEAR (a wall neighbour or the domain edge) take a slow path that reads the
+// neighbour flags, bounces populations back at walls (halfway bounce-back, with the Ladd
+// momentum term at moving walls) and wraps periodically at the domain edge.
+#pragma once
+#include <type_traits>
+#include <utility>
+#include "common.hpp"
+
+namespace lbdem {
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&fn) {
+  if constexpr (I < N) {
+    fn(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(fn);
+  }
+}
+
+// accumulate +v / -v / nothing according to a lattice velocity component
+template <int a, typename T>
+__device__ __forceinline__ void acc(T &s, T v) {
+  if constexpr (a == 1) s += v;
+  else if constexpr (a == -1) s -= v;
+}
+template <int a, typename T>
+__device__ __forceinline__ T sgn(T v) {
+  if constexpr (a == 1) return v;
+  else return -v;
+}
+// c_q . (x, y, z) without multiplications by 0 (D3Q19 has at most two non-zero components)
+template <int q, typename T>
+__device__ __forceinline__ T cdot(T x, T y, T z) {
+  constexpr int a = kCX[q], b = kCY[q], c = kCZ[q];
+  if constexpr (a != 0 && b != 0) return sgn<a>(x) + sgn<b>(y);
+  else if constexpr (a != 0 && c != 0) return sgn<a>(x) + sgn<c>(z);
+  else if constexpr (b != 0 && c != 0) return sgn<b>(y) + sgn<c>(z);
+  else if constexpr (a != 0) return sgn<a>(x);
+  else if constexpr (b != 0) return sgn<b>(y);
+  else if constexpr (c != 0) return sgn<c>(z);
+  else return T(0);
+}
+
+constexpr int kMaskSolid = LBDEM_FLAG_WALL | LBDEM_FLAG_MOVING;
+
+// Post-stream (shifted) populations of cell c after t steps, t parity `odd`.
+template <typename S>
+__device__ __forceinline__ void gather(const KParams &p, const S *__restrict__ F, int c, uint8_t fl,
+                                       bool odd, typename Store<S>::T (&f)[Q]) {
+  const size_t N = (size_t)p.ncell;
+  if (!odd) {
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      f[q] = Store<S>::ld(F + q * N + c);
+    });
+  } else if (!(fl & LBDEM_FLAG_NEAR)) {
+    const int sx = p.ny * p.nz, sy = p.nz;
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      const int off = kCX[q] * sx + kCY[q] * sy + kCZ[q];
+      f[q] = Store<S>::ld(F + kOPP[q] * N + (c - off));
+    });
+  } else {
+    int i, j, k;
+    cell_ijk(p, c, i, j, k);
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      if constexpr (q == 0) {
+        f[0] = Store<S>::ld(F + c);
+      } else {
+        const int n = nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
+        if (p.flags[n] & kMaskSolid) f[q] = Store<S>::ld(F + q * N + c);  // bounced back last step
+        else f[q] = Store<S>::ld(F + kOPP[q] * N + n);
+      }
+    });
+  }
+}
+
+// Tenneti et al. (2011) drag weight of the particle-equivalence IBM, in lattice units.
+// Follows EqIMBlattice3D.compute_weight / weight_coefficient (src/unlbdem/eqlattice.py:200-238):
+// R = (3 eps / 4 pi)^(1/3), d = 2R, Re = (1-eps) d |u_s - u| / nu, W = 3 pi d nu (1-eps) C_d.
+template <typename T>
+__device__ __forceinline__ T tenneti_weight(T eps, T slip, T nu) {
+  const T pi = T(3.14159265358979323846);
+  const T R = cbrt(T(3) * eps / (T(4) * pi));
+  const T d = T(2) * R;
+  const T om = T(1) - eps;
+  const T Re = om * d * slip / nu;
+  T Cd = T(0);
+  if (om > T(1e-9)) {
+    const T om2 = om * om, om3 = om2 * om;
+    const T Cd0 = T(1) + T(0.15) * pow(Re, T(0.687));
+    const T A = T(5.81) * eps / om3 + T(0.48) * cbrt(eps) / (om3 * om);
+    const T e3 = eps * eps * eps;
+    const T B = e3 * Re * (T(0.95) + T(0.61) * e3 / om2);
+    Cd = om * (Cd0 / om3 + A + B);
+  }
+  return T(3) * pi * d * nu * om * Cd;
+}
+
+// BGK / Smagorinsky / PSC / EqIBM / VANS collision of one cell, in shifted variables.
+// f: pre-collision (in), post-collision (out).  Returns rho (for the moving-wall term).
+template <int COLL, int COUP, typename T>
+__device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int k, T (&f)[Q]) {
+  T dr = T(0), jx = T(0), jy = T(0), jz = T(0);
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    dr += f[q];
+    acc<kCX[q]>(jx, f[q]);
+    acc<kCY[q]>(jy, f[q]);
+    acc<kCZ[q]>(jz, f[q]);
+  });
+  const T rho = T(p.rho0) + dr;
+  const T inv = T(1) / rho;
+  T ux = jx * inv, uy = jy * inv, uz = jz * inv;
+
+  T fvf = T(1);  // fluid volume fraction in the VANS equilibrium
+  T Fx = T(0), Fy = T(0), Fz = T(0);
+  if constexpr (COUP == LBDEM_COUPLE_VANS) {
+    // macroscopic velocity with the previous force / volume fraction
+    // (compute_rhof_vel, src/unlbdem/vanslattice.py:412-440)
+    const T fvfp = T(1) - (T)p.volfrac_prev[c];
+    const T hx = (T)p.hforce_prev[3 * c], hy = (T)p.hforce_prev[3 * c + 1], hz = (T)p.hforce_prev[3 * c + 2];
+    const T s = T(1) / (fvfp * rho);
+    ux = (ux + T(0.5) * hx) * s;
+    uy = (uy + T(0.5) * hy) * s;
+    uz = (uz + T(0.5) * hz) * s;
+    fvf = T(1) - (T)p.volfrac[c];
+    Fx = (T)p.hforce[3 * c];
+    Fy = (T)p.hforce[3 * c + 1];
+    Fz = (T)p.hforce[3 * c + 2];
+  }
+
+  // relaxation frequency
+  T omega;
+  if constexpr (COUP == LBDEM_COUPLE_VANS) {
+    // compute_omega (src/unlbdem/vanslattice.py:323-358)
+    const T svf = T(1) - fvf;
+    const T g = T(1) + T(1.25) * svf * T(0.64) / (T(1) - svf);
+    const T nu_eff = T(p.nu_lu) * g * g;
+    omega = T(1) / (T(3) * nu_eff + T(0.5));
+  } else if constexpr (COLL == LBDEM_COLL_BGK) {
+    omega = T(p.omega0);
+  } else {
+    // non-equilibrium momentum flux; sum_q c_a c_b feq = rho/3 delta_ab + rho u_a u_b holds
+    // exactly for D3Q19, so Pi^neq = sum c c f - that closed form
+    // (compute_strain_rate, src/lbm3d/lbm_solver3d.py:460-498)
+    T pxx = T(0), pyy = T(0), pzz = T(0), pxy = T(0), pxz = T(0), pyz = T(0);
+    static_for<1, Q>([&](auto qc) {
+      constexpr int q = qc;
+      if constexpr (kCX[q] != 0) pxx += f[q];
+      if constexpr (kCY[q] != 0) pyy += f[q];
+      if constexpr (kCZ[q] != 0) pzz += f[q];
+      if constexpr (kCX[q] * kCY[q] == 1) pxy += f[q];
+      if constexpr (kCX[q] * kCY[q] == -1) pxy -= f[q];
+      if constexpr (kCX[q] * kCZ[q] == 1) pxz += f[q];
+      if constexpr (kCX[q] * kCZ[q] == -1) pxz -= f[q];
+      if constexpr (kCY[q] * kCZ[q] == 1) pyz += f[q];
+      if constexpr (kCY[q] * kCZ[q] == -1) pyz -= f[q];
+    });
+    T er, eux, euy, euz;  // state whose equilibrium defines Pi^neq
+    T omega_old = T(0);
+    if constexpr (COLL == LBDEM_COLL_SMAG_LAGGED) {
+      const size_t N = (size_t)p.ncell;
+      er = (T)p.lag[c];
+      eux = (T)p.lag[N + c];
+      euy = (T)p.lag[2 * N + c];
+      euz = (T)p.lag[3 * N + c];
+      omega_old = (T)p.lag[4 * N + c];
+    } else {
+      er = rho;
+      eux = ux;
+      euy = uy;
+      euz = uz;
+    }
+    const T edr3 = (er - T(p.rho0)) * T(1.0 / 3.0);
+    const T nxx = pxx - (edr3 + er * eux * eux);
+    const T nyy = pyy - (edr3 + er * euy * euy);
+    const T nzz = pzz - (edr3 + er * euz * euz);
+    const T nxy = pxy - er * eux * euy;
+    const T nxz = pxz - er * eux * euz;
+    const T nyz = pyz - er * euy * euz;
+    const T Pi = sqrt(T(2) * (nxx * nxx + nyy * nyy + nzz * nzz + T(2) * (nxy * nxy + nxz * nxz + nyz * nyz)));
+    if constexpr (COLL == LBDEM_COLL_SMAG_LAGGED) {
+      // rate = omega_old Pi / (2 cs^2 rho); tau = tau0 + Cs^2 rate / cs^2
+      // (computeOmega, src/lbm3d/lbm_solver3d.py:500-513)
+      const T rate = omega_old * Pi * T(1.5) * inv;
+      omega = T(1) / (T(p.tau0) + T(3) * T(p.cs2_smag) * rate);
+      const size_t N = (size_t)p.ncell;
+      p.lag[c] = (float)rho;
+      p.lag[N + c] = (float)ux;
+      p.lag[2 * N + c] = (float)uy;
+      p.lag[3 * N + c] = (float)uz;
+      p.lag[4 * N + c] = (float)omega;
+    } else {
+      // tau = tau0 + 9/2 Cs^2 Pi / (rho tau)  =>  tau = (tau0 + sqrt(tau0^2 + 18 Cs^2 Pi/rho)) / 2
+      const T tau0 = T(p.tau0);
+      omega = T(2) / (tau0 + sqrt(tau0 * tau0 + T(18) * T(p.cs2_smag) * Pi * inv));
+    }
+  }
+
+  const T usq = ux * ux + uy * uy + uz * uz;
+  const T rf = rho * fvf;
+
+  // solid coupling weight B and solid velocity
+  T B = T(0), usx = T(0), usy = T(0), usz = T(0);
+  bool solid = false;
+  if constexpr (COUP == LBDEM_COUPLE_PSC) {
+    const uint32_t key = p.psc_key[c];
+    if (key != 0u) {
+      solid = true;
+      const int g = 0x7FFFFF - (int)(key & 0x7FFFFFu);
+      const float *G = p.psc_grain + 12 * g;
+      const T rx = T(i) - (T)G[0], ry = T(j) - (T)G[1], rz = T(k) - (T)G[2];
+      const T wx = (T)G[6], wy = (T)G[7], wz = (T)G[8];
+      usx = (T)G[3] + (wy * rz - wz * ry);
+      usy = (T)G[4] + (wz * rx - wx * rz);
+      usz = (T)G[5] + (wx * ry - wy * rx);
+      if (key & (1u << 30)) {
+        B = T(1);
+      } else {
+        // B = eps (tau - 1/2) / ((1 - eps) + (tau - 1/2))  (src/lbdem/psclattice3d.py:144-155)
+        const T eps = T((key >> 23) & 0x7Fu) * T(1.0 / 125.0);
+        const T th = T(p.tau0) - T(0.5);
+        B = eps * th / ((T(1) - eps) + th);
+      }
+    }
+  } else if constexpr (COUP == LBDEM_COUPLE_EQIBM) {
+    const T eps = (T)p.volfrac[c];
+    if (eps > T(0)) {
+      solid = true;
+      usx = (T)p.velsolid[3 * c];
+      usy = (T)p.velsolid[3 * c + 1];
+      usz = (T)p.velsolid[3 * c + 2];
+      const T dx = usx - ux, dy = usy - uy, dz = usz - uz;
+      B = tenneti_weight<T>(eps, sqrt(dx * dx + dy * dy + dz * dz), T(p.nu_lu));
+    }
+  }
+
+  const bool forced = (COUP != LBDEM_COUPLE_VANS) && p.has_force;
+  const T F3x = T(3) * rho * T(p.fx), F3y = T(3) * rho * T(p.fy), F3z = T(3) * rho * T(p.fz);
+  const T vans_pref = T(1) - T(0.5) * omega;
+
+  if (!solid) {
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      const T w = T(kW[q]);
+      const T cu = cdot<q>(ux, uy, uz);
+      const T feq = w * (dr + rf * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq));
+      T nf = f[q] - omega * (f[q] - feq);
+      if constexpr (COUP == LBDEM_COUPLE_VANS) {
+        // F_q = w (1 - omega/2) (3c - 3u + 9 (c.u) c) . F  (src/unlbdem/vanslattice.py:278-318)
+        const T cF = cdot<q>(Fx, Fy, Fz);
+        nf += w * vans_pref * (T(3) * cF - T(3) * (ux * Fx + uy * Fy + uz * Fz) + T(9) * cu * cF);
+      } else {
+        if (forced) nf += w * cdot<q>(F3x, F3y, F3z);
+      }
+      f[q] = nf;
+    });
+  } else {
+    // weighted collision fpc = f + B Omega_s + (1-B) Omega_f with
+    // Omega_s = f_qbar - feq_qbar + feq_s,q - f_q,  Omega_f = -omega (f_q - feq_q)
+    // (src/lbdem/psclattice3d.py:240-261, src/unlbdem/eqlattice.py:272-292)
+    const T ussq = usx * usx + usy * usy + usz * usz;
+    T hx = T(0), hy = T(0), hz = T(0);
+    const T oB = T(1) - B;
+    {
+      // rest population: its own opposite
+      const T feq0 = T(kW[0]) * (dr - rho * T(1.5) * usq);
+      const T feqs0 = T(kW[0]) * (dr - rho * T(1.5) * ussq);
+      const T Os = feqs0 - feq0;
+      f[0] = f[0] + B * Os - oB * omega * (f[0] - feq0);
+    }
+    static_for<1, Q>([&](auto qc) {
+      constexpr int q = qc;
+      if constexpr (kOPP[q] > q) {
+        constexpr int r = kOPP[q];
+        const T w = T(kW[q]);
+        const T cu = cdot<q>(ux, uy, uz);
+        const T cus = cdot<q>(usx, usy, usz);
+        const T sym = dr + rho * (T(4.5) * cu * cu - T(1.5) * usq);
+        const T syms = dr + rho * (T(4.5) * cus * cus - T(1.5) * ussq);
+        const T feq_q = w * (sym + T(3) * rho * cu), feq_r = w * (sym - T(3) * rho * cu);
+        const T fs_q = w * (syms + T(3) * rho * cus), fs_r = w * (syms - T(3) * rho * cus);
+        const T fq = f[q], fr = f[r];
+        const T Os_q = fr - feq_r + fs_q - fq;
+        const T Os_r = fq - feq_q + fs_r - fr;
+        T nq = fq + B * Os_q - oB * omega * (fq - feq_q);
+        T nr = fr + B * Os_r - oB * omega * (fr - feq_r);
+        if (forced) {
+          const T cF = cdot<q>(F3x, F3y, F3z);
+          nq += oB * w * cF;
+          nr -= oB * w * cF;
+        }
+        f[q] = nq;
+        f[r] = nr;
+        // hydroforce -= B Omega_s c (c_r = -c_q)
+        const T d = B * (Os_q - Os_r);
+        acc<kCX[q]>(hx, -d);
+        acc<kCY[q]>(hy, -d);
+        acc<kCZ[q]>(hz, -d);
+      }
+    });
+    if constexpr (COUP == LBDEM_COUPLE_PSC) {
+      p.psc_force[3 * c] = (float)hx;
+      p.psc_force[3 * c + 1] = (float)hy;
+      p.psc_force[3 * c + 2] = (float)hz;
+    }
+  }
+  return rho;
+}
+
+// Moving-wall (Ladd) momentum term for a population leaving along c_q into wall cell m:
+// the bounced value is f*_q - 6 w_q rho (c_q . u_w)  (src/lbm3d/lbm_solver3d.py:221-224).
+template <int q, typename T>
+__device__ __forceinline__ T ladd_term(const KParams &p, int m, T rho) {
+  const T uwx = (T)p.vel[3 * m], uwy = (T)p.vel[3 * m + 1], uwz = (T)p.vel[3 * m + 2];
+  return T(6) * T(kW[q]) * rho * cdot<q>(uwx, uwy, uwz);
+}
+
+template <typename S, int COLL, int COUP, bool ODD>
+__global__ __launch_bounds__(256) void k_step(const KParams p) {
+  using T = typename Store<S>::T;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= p.ncell) return;
+  const uint8_t fl = p.flags[c];
+  if (!(fl & LBDEM_FLAG_ACTIVE)) return;
+  S *__restrict__ F = reinterpret_cast<S *>(p.f);
+  const size_t N = (size_t)p.ncell;
+  const bool near = fl & LBDEM_FLAG_NEAR;
+
+  T f[Q];
+  gather<S>(p, F, c, fl, ODD, f);
+
+  int i = 0, j = 0, k = 0;
+  if (COUP == LBDEM_COUPLE_PSC || near) cell_ijk(p, c, i, j, k);
+  const T rho = collide<COLL, COUP, T>(p, c, i, j, k, f);
+
+  if (!near) {
+    if constexpr (!ODD) {
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        Store<S>::st(F + kOPP[q] * N + c, f[q]);
+      });
+    } else {
+      const int sx = p.ny * p.nz, sy = p.nz;
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        const int off = kCX[q] * sx + kCY[q] * sy + kCZ[q];
+        Store<S>::st(F + q * N + (c + off), f[q]);
+      });
+    }
+  } else {
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      if constexpr (q == 0) {
+        Store<S>::st(F + c, f[0]);
+      } else {
+        const int m = nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q]);
+        const uint8_t fm = p.flags[m];
+        if (fm & kMaskSolid) {
+          T v = f[q];
+          if (fm & LBDEM_FLAG_MOVING) v -= ladd_term<q, T>(p, m, rho);
+          Store<S>::st(F + kOPP[q] * N + c, v);
+        } else if constexpr (!ODD) {
+          Store<S>::st(F + kOPP[q] * N + c, f[q]);
+        } else {
+          Store<S>::st(F + q * N + m, f[q]);
+        }
+      }
+    });
+  }
+}
+
+using StepKernel = void (*)(const KParams);
+
+// f <- feq(rho, u) at active cells, natural layout; lag state initialised
+template <typename S, int COUP>
+__global__ __launch_bounds__(256) void k_init(const KParams p) {
+  using T = typename Store<S>::T;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= p.ncell) return;
+  if (!(p.flags[c] & LBDEM_FLAG_ACTIVE)) return;
+  S *__restrict__ F = reinterpret_cast<S *>(p.f);
+  const size_t N = (size_t)p.ncell;
+  const T rho = (T)p.rho_out[c];
+  const T ux = (T)p.vel[3 * c], uy = (T)p.vel[3 * c + 1], uz = (T)p.vel[3 * c + 2];
+  const T dr = rho - T(p.rho0);
+  const T usq = ux * ux + uy * uy + uz * uz;
+  T fvf = T(1);
+  if (COUP == LBDEM_COUPLE_VANS && p.volfrac) fvf = T(1) - (T)p.volfrac[c];
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    const T cu = cdot<q>(ux, uy, uz);
+    Store<S>::st(F + q * N + c, T(kW[q]) * (dr + rho * fvf * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq)));
+  });
+  if (p.lag) {
+    p.lag[c] = (float)rho;
+    p.lag[N + c] = (float)ux;
+    p.lag[2 * N + c] = (float)uy;
+    p.lag[3 * N + c] = (float)uz;
+    p.lag[4 * N + c] = (float)p.omega0;
+  }
+}
+
+// rho, u of the post-stream state (compute_rho_vel / compute_rhof_vel)
+template <typename S, int COUP>
+__global__ __launch_bounds__(256) void k_macros(const KParams p) {
+  using T = typename Store<S>::T;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= p.ncell) return;
+  const uint8_t fl = p.flags[c];
+  if (!(fl & LBDEM_FLAG_ACTIVE)) return;
+  if (COUP != LBDEM_COUPLE_VANS && !(fl & LBDEM_FLAG_FLUID)) return;
+  if (p.mask && !(p.mask[c] > 0.0f)) return;
+  const S *F = reinterpret_cast<const S *>(p.f);
+  T f[Q];
+  gather<S>(p, F, c, fl, p.t_odd != 0, f);
+  T dr = T(0), jx = T(0), jy = T(0), jz = T(0);
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    dr += f[q];
+    acc<kCX[q]>(jx, f[q]);
+    acc<kCY[q]>(jy, f[q]);
+    acc<kCZ[q]>(jz, f[q]);
+  });
+  const T rho = T(p.rho0) + dr;
+  T ux = jx / rho, uy = jy / rho, uz = jz / rho;
+  if constexpr (COUP == LBDEM_COUPLE_VANS) {
+    // the force / volume fraction of the step that produced this state
+    const T fvf = T(1) - (T)p.volfrac[c];
+    const T s = T(1) / (fvf * rho);
+    ux = (ux + T(0.5) * (T)p.hforce[3 * c]) * s;
+    uy = (uy + T(0.5) * (T)p.hforce[3 * c + 1]) * s;
+    uz = (uz + T(0.5) * (T)p.hforce[3 * c + 2]) * s;
+  }
+  p.rho_out[c] = (float)rho;
+  p.vel_out[3 * c] = (float)ux;
+  p.vel_out[3 * c + 1] = (float)uy;
+  p.vel_out[3 * c + 2] = (float)uz;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_read_f(const KParams p) {
+  using T = typename Store<S>::T;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= p.ncell) return;
+  const size_t N = (size_t)p.ncell;
+  const uint8_t fl = p.flags[c];
+  T f[Q];
+  const bool act = fl & LBDEM_FLAG_ACTIVE;
+  if (act) gather<S>(p, reinterpret_cast<const S *>(p.f), c, fl, p.t_odd != 0, f);
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    const double v = act ? (double)f[q] + kW[q] * p.rho0 : 0.0;
+    if (p.out_f64) reinterpret_cast<double *>(p.out)[q * N + c] = v;
+    else reinterpret_cast<float *>(p.out)[q * N + c] = (float)v;
+  });
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_write_f(const KParams p) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= p.ncell) return;
+  if (!(p.flags[c] & LBDEM_FLAG_ACTIVE)) return;
+  const size_t N = (size_t)p.ncell;
+  S *F = reinterpret_cast<S *>(p.f);
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    const double v = p.out_f64 ? reinterpret_cast<const double *>(p.in)[q * N + c]
+                               : (double)reinterpret_cast<const float *>(p.in)[q * N + c];
+    Store<S>::st(F + q * N + c, (typename Store<S>::T)(v - kW[q] * p.rho0));
+  });
+}
+
+// per-workgroup FP64 partial sums: rho, rho u (3), rho |u|^2 / 2
+template <typename S>
+__global__ __launch_bounds__(256) void k_reduce_partial(const KParams p) {
+  using T = typename Store<S>::T;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  double s[5] = {0, 0, 0, 0, 0};
+  if (c < p.ncell) {
+    const uint8_t fl = p.flags[c];
+    if (fl & LBDEM_FLAG_ACTIVE) {
+      T f[Q];
+      gather<S>(p, reinterpret_cast<const S *>(p.f), c, fl, p.t_odd != 0, f);
+      double dr = 0, jx = 0, jy = 0, jz = 0;
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        const double v = (double)f[q];
+        dr += v;
+        acc<kCX[q]>(jx, v);
+        acc<kCY[q]>(jy, v);
+        acc<kCZ[q]>(jz, v);
+      });
+      const double rho = p.rho0 + dr;
+      s[0] = rho;
+      s[1] = jx;
+      s[2] = jy;
+      s[3] = jz;
+      s[4] = 0.5 * (jx * jx + jy * jy + jz * jz) / rho;
+    }
+  }
+  __shared__ double sh[4][5];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    const double v = wave_sum(s[m]);
+    if (lane == 0) sh[wv][m] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int m = threadIdx.x;
+    p.work[(size_t)blockIdx.x * 5 + m] = ((sh[0][m] + sh[1][m]) + (sh[2][m] + sh[3][m]));
+  }
+}
+
+}  // namespace lbdem

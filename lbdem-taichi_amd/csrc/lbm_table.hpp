@@ -1,0 +1,62 @@
+// Per-storage-type kernel tables (one translation unit per storage type).
+#pragma once
+#include "common.hpp"
+
+namespace lbdem {
+
+using KernelFn = void (*)(const KParams);
+
+struct LbmKernels {
+  KernelFn step[3][4][2];  // [collision][coupling][odd]; nullptr = unsupported combination
+  KernelFn init[4];        // [coupling]
+  KernelFn macros[4];      // [coupling]
+  KernelFn read_f;
+  KernelFn write_f;
+  KernelFn reduce_partial;
+};
+
+const LbmKernels &lbm_kernels_f32();
+const LbmKernels &lbm_kernels_f16();
+const LbmKernels &lbm_kernels_bf16();
+const LbmKernels &lbm_kernels_f64();
+
+}  // namespace lbdem
+
+// Builds the table for storage type S (expanded once in each lbm_<type>.hip).
+#define LBDEM_DEFINE_LBM_TABLE(S, NAME)                                                        \
+  namespace lbdem {                                                                             \
+  const LbmKernels &NAME() {                                                                    \
+    static const LbmKernels t = [] {                                                            \
+      LbmKernels k{};                                                                           \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false>;               \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true>;                \
+      k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false>; \
+      k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true>;  \
+      k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false>;   \
+      k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true>;    \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false>;                 \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true>;                  \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false>;             \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true>;              \
+      k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false>; \
+      k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true>;  \
+      k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false>;   \
+      k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true>;    \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false>;               \
+      k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true>;                \
+      k.init[LBDEM_COUPLE_NONE] = k_init<S, LBDEM_COUPLE_NONE>;                                 \
+      k.init[LBDEM_COUPLE_PSC] = k_init<S, LBDEM_COUPLE_NONE>;                                  \
+      k.init[LBDEM_COUPLE_EQIBM] = k_init<S, LBDEM_COUPLE_NONE>;                                \
+      k.init[LBDEM_COUPLE_VANS] = k_init<S, LBDEM_COUPLE_VANS>;                                 \
+      k.macros[LBDEM_COUPLE_NONE] = k_macros<S, LBDEM_COUPLE_NONE>;                             \
+      k.macros[LBDEM_COUPLE_PSC] = k_macros<S, LBDEM_COUPLE_NONE>;                              \
+      k.macros[LBDEM_COUPLE_EQIBM] = k_macros<S, LBDEM_COUPLE_NONE>;                            \
+      k.macros[LBDEM_COUPLE_VANS] = k_macros<S, LBDEM_COUPLE_VANS>;                             \
+      k.read_f = k_read_f<S>;                                                                   \
+      k.write_f = k_write_f<S>;                                                                 \
+      k.reduce_partial = k_reduce_partial<S>;                                                   \
+      return k;                                                                                 \
+    }();                                                                                        \
+    return t;                                                                                   \
+  }                                                                                             \
+  }

@@ -1,0 +1,282 @@
+"""D3Q19 lattice with BGK + Smagorinsky collision (API of src/lbm3d/lbm_solver3d.py).
+
+`BasicLattice3D` keeps the reference's constructor, constants, fields and step methods
+(`initialize`, `collide`, `stream`), but a time step is one fused HIP kernel: `collide()`
+enqueues the collide-stream update of the populations (in place, AA access pattern) and
+`stream()` has nothing left to do.  Macroscopic `rho` / `vel` are produced lazily from the
+populations when they are read.  The reference's `feq` / `fpc` scratch fields do not exist;
+`feq` is reconstructed on request.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _hip
+from .._field import Field
+from ..runtime import config
+from .lbmutils import CellType, UnitConverter
+
+_TYPE_BITS = (CellType.FLUID | CellType.INTERFACE | CellType.GAS | CellType.OBSTACLE | CellType.FREE_SLIP |
+              CellType.VEL_LADD | CellType.VEL_ZOUHE | CellType.VEL_EXIT | CellType.Pre_ZOUHE |
+              CellType.VEL_INLET_LADD)
+
+
+class _PopulationField:
+    """`lattice.f`: unshifted populations of the post-stream state, shape (Nx, Ny, Nz, 19)."""
+
+    def __init__(self, lattice):
+        self._lat = lattice
+        self.shape = (lattice.Nx, lattice.Ny, lattice.Nz)
+        self.n = 19
+
+    def to_torch(self, dtype=torch.float64):
+        L = self._lat
+        L._prepare()
+        out = torch.empty((19, L.N), dtype=dtype, device=L.device)
+        _hip.check(_hip.lib().lbdem_lbm_read_f(L._lat_c, L._args_c(), _hip.ptr(out), int(dtype == torch.float64),
+                                               _hip.stream()), "lbdem_lbm_read_f")
+        return out
+
+    def to_numpy(self):
+        L = self._lat
+        return self.to_torch().T.reshape(L.Nx, L.Ny, L.Nz, 19).cpu().numpy()
+
+    def from_numpy(self, arr):
+        L = self._lat
+        arr = np.asarray(arr, dtype=np.float64)
+        if arr.shape != (L.Nx, L.Ny, L.Nz, 19):
+            raise ValueError(f"expected shape {(L.Nx, L.Ny, L.Nz, 19)}, got {arr.shape}")
+        L._prepare()
+        src = torch.from_numpy(np.ascontiguousarray(arr.reshape(L.N, 19).T)).to(L.device)
+        _hip.check(_hip.lib().lbdem_lbm_write_f(L._lat_c, L._args_c(), _hip.ptr(src), 1, _hip.stream()),
+                   "lbdem_lbm_write_f")
+        L.t = 0
+        L._macro_t = -1
+
+    def __getitem__(self, idx):
+        return self.to_numpy()[idx]
+
+
+class BasicLattice3D:
+    """LBM D3Q19, cs^2 = 1/3, w0 = 1/3, w1-6 = 1/18, w7-18 = 1/36 (reference lbm_solver3d.py:18-97)."""
+    D = 3
+    Q = 19
+    qinv = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17], dtype=np.int32)
+    qsyx = np.array([0, 2, 1, 3, 4, 5, 6, 10, 9, 8, 7, 14, 13, 12, 11, 15, 16, 17, 18], dtype=np.int32)
+    qsyy = np.array([0, 1, 2, 4, 3, 5, 6, 9, 10, 7, 8, 11, 12, 13, 14, 18, 17, 16, 15], dtype=np.int32)
+    qsyz = np.array([0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16], dtype=np.int32)
+    c = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                  [1, 1, 0], [-1, -1, 0], [1, -1, 0], [-1, 1, 0], [1, 0, 1], [-1, 0, -1], [1, 0, -1],
+                  [-1, 0, 1], [0, 1, 1], [0, -1, -1], [0, 1, -1], [0, -1, 1]], dtype=np.float64)
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float64)
+    velmax = (2.0 / 3.0) ** 0.5
+    cssq = 1.0 / 3.0
+    SmagorinskyConstant = 0.1  # read at every step, so drivers may change it (e.g. 0.17)
+
+    # cells that collide() / stream() skip (lbm_solver3d.py:183)
+    _skip_types = CellType.OBSTACLE | CellType.VEL_LADD | CellType.FREE_SLIP | CellType.VEL_INLET_LADD
+    _coupling = _hip.COUPLE_NONE
+
+    def __init__(self, Nx: int, Ny: int, Nz: int, omega: float, dx: float, dt: float, rho: float,
+                 storage=None, smagorinsky=None):
+        _hip.require_gpu()
+        self.Nx, self.Ny, self.Nz = int(Nx), int(Ny), int(Nz)
+        self.N = self.Nx * self.Ny * self.Nz
+        self.omega0 = float(omega)
+        self.storage = storage or config.storage
+        if self.storage not in _hip.STORE:
+            raise ValueError(f"unknown storage {self.storage!r}")
+        self.smagorinsky = smagorinsky or config.smagorinsky
+        self.device = config.torch_device
+        shp = (self.Nx, self.Ny, self.Nz)
+        dev = self.device
+
+        self.CT = Field(shp, torch.int32, dev)
+        self.CT.fill(CellType.FLUID)
+        self.rho = Field(shp, torch.float32, dev, refresh=self._refresh_macros)
+        self.rho.fill(1.0)
+        self.vel = Field(shp, torch.float32, dev, n=3, refresh=self._refresh_macros)
+        self.unit = UnitConverter(dx, dt, rho)
+
+        self._f = torch.zeros(19 * self.N, dtype=_hip.STORE_DTYPE[self.storage], device=dev)
+        self._flags = torch.zeros(self.N, dtype=torch.uint8, device=dev)
+        self._lag = None
+        if self._collision() == _hip.COLL_SMAG_LAGGED:
+            self._lag = torch.zeros(5 * self.N, dtype=torch.float32, device=dev)
+            self.omega = Field(shp, torch.float32, dev, tensor=self._lag[4 * self.N:].view(shp))
+        else:
+            self.omega = Field(shp, torch.float32, dev)
+        self.omega.fill(self.omega0)
+        self.f = _PopulationField(self)
+
+        self.t = 0                 # steps taken (AA parity of the population array)
+        self._macro_t = 0          # step at which rho/vel were last refreshed
+        self._ct_version = -1
+        self._lat_c = _hip.Lattice()
+        self._args = _hip.StepArgs()
+        self._sync_lattice_struct()
+
+    # ----------------------------------------------------------------------------------
+    # configuration of the fused kernel
+    # ----------------------------------------------------------------------------------
+    def _collision(self):
+        return _hip.COLL_SMAG_LAGGED if self.smagorinsky == "lagged" else _hip.COLL_SMAG_FIXED
+
+    def _body_force(self):
+        return (0.0, 0.0, 0.0)
+
+    def _sync_lattice_struct(self):
+        L = self._lat_c
+        L.nx, L.ny, L.nz = self.Nx, self.Ny, self.Nz
+        L.storage = _hip.STORE[self.storage]
+        L.collision = self._collision()
+        L.coupling = self._coupling
+        L.omega0 = self.omega0
+        L.cs_smag = float(BasicLattice3D.SmagorinskyConstant)
+        L.rho0 = 1.0
+        fx, fy, fz = self._body_force()
+        L.force[0], L.force[1], L.force[2] = float(fx), float(fy), float(fz)
+
+    def _coupling_args(self, a):
+        """Subclasses fill the coupling members of the step arguments."""
+
+    def _args_c(self):
+        a = self._args
+        a.f = self._f.data_ptr()
+        a.flags = self._flags.data_ptr()
+        a.vel = self.vel.dev.data_ptr()
+        a.lag = self._lag.data_ptr() if self._lag is not None else None
+        a.t = self.t
+        a.nu_lu = (1.0 / self.omega0 - 0.5) / 3.0
+        self._coupling_args(a)
+        return ctypes.byref(a)
+
+    # ----------------------------------------------------------------------------------
+    # cell flags: CellType -> one byte per cell (active / near-wall / wall / moving / fluid)
+    # ----------------------------------------------------------------------------------
+    def _build_flags(self):
+        ct = self.CT.to_numpy().astype(np.int64)
+        kind = ct & _TYPE_BITS
+        supported = (kind == CellType.FLUID) | (kind == CellType.OBSTACLE) | (kind == CellType.VEL_LADD)
+        if not supported.all():
+            bad = np.unique(kind[~supported])
+            raise NotImplementedError(
+                f"cell types {bad.tolist()} are not supported by the fused kernel (supported: FLUID, "
+                "OBSTACLE, VEL_LADD; wet-node and free-slip boundaries are not implemented)")
+        active = (ct & self._skip_types) == 0
+        wall = kind == CellType.OBSTACLE
+        moving = kind == CellType.VEL_LADD
+        fluid = kind == CellType.FLUID
+        near = np.zeros_like(active)
+        for cx, cy, cz in self.c[1:].astype(int):
+            near |= ~np.roll(active, shift=(-cx, -cy, -cz), axis=(0, 1, 2))
+        near[0, :, :] = near[-1, :, :] = True
+        near[:, 0, :] = near[:, -1, :] = True
+        near[:, :, 0] = near[:, :, -1] = True
+        near &= active
+        flags = (active * _hip.FLAG_ACTIVE | near * _hip.FLAG_NEAR | wall * _hip.FLAG_WALL |
+                 moving * _hip.FLAG_MOVING | fluid * _hip.FLAG_FLUID).astype(np.uint8)
+        self._flags.copy_(torch.from_numpy(flags.reshape(-1)))
+        self._ct_version = self.CT.version
+        self._active_np = active
+
+    def _prepare(self):
+        """Bring device inputs up to date before a kernel launch."""
+        if self.CT.version != self._ct_version:
+            self._build_flags()
+        self.CT.sync()
+        self.vel.sync()
+        self.rho.sync()
+        self.omega.sync()
+        self._sync_lattice_struct()
+
+    def _refresh_macros(self):
+        if self._macro_t == self.t:
+            return
+        self._prepare()
+        _hip.check(_hip.lib().lbdem_lbm_macros(self._lat_c, self._args_c(), _hip.ptr(self.rho.dev),
+                                               _hip.ptr(self.vel.dev), None, _hip.stream()), "lbdem_lbm_macros")
+        self._macro_t = self.t
+        self.rho.touch()
+        self.vel.touch()
+
+    # ----------------------------------------------------------------------------------
+    # reference step API
+    # ----------------------------------------------------------------------------------
+    def initialize(self):
+        """f <- feq(rho, vel) at collided cells (lbm_solver3d.py:157-171)."""
+        self._prepare()
+        self.t = 0
+        _hip.check(_hip.lib().lbdem_lbm_init(self._lat_c, self._args_c(), _hip.ptr(self.rho.dev), _hip.stream()),
+                   "lbdem_lbm_init")
+        self._macro_t = 0
+        if self._lag is not None:
+            self.omega.touch()
+
+    def collide(self):
+        """One fused collide + stream step (lbm_solver3d.py:176-250)."""
+        self._prepare()
+        _hip.check(_hip.lib().lbdem_lbm_step(self._lat_c, self._args_c(), _hip.stream()), "lbdem_lbm_step")
+        self.t += 1
+        if self._lag is not None:
+            self.omega.touch()
+
+    def stream(self):
+        """Streaming is fused into collide(); no wet-node boundaries are supported, so
+        there is nothing left to do here (lbm_solver3d.py:197-250)."""
+
+    def step(self, n=1):
+        for _ in range(n):
+            self.collide()
+            self.stream()
+
+    # ----------------------------------------------------------------------------------
+    # diagnostics
+    # ----------------------------------------------------------------------------------
+    def _reduce(self):
+        self._prepare()
+        work = torch.empty(int(_hip.lib().lbdem_reduce_work_doubles(self._lat_c)), dtype=torch.float64,
+                           device=self.device)
+        out = torch.empty(5, dtype=torch.float64, device=self.device)
+        _hip.check(_hip.lib().lbdem_lbm_reduce(self._lat_c, self._args_c(), _hip.ptr(work), _hip.ptr(out),
+                                               _hip.stream()), "lbdem_lbm_reduce")
+        return out.cpu().numpy()
+
+    def get_total_mass(self) -> float:
+        """FP64 sum of rho over collided cells."""
+        return float(self._reduce()[0])
+
+    def get_total_momentum(self):
+        """FP64 sum of rho*u over collided cells."""
+        return self._reduce()[1:4].copy()
+
+    def get_kinetic_energy(self) -> float:
+        """FP64 sum of rho|u|^2/2 over collided cells."""
+        return float(self._reduce()[4])
+
+    @property
+    def feq(self):
+        """Equilibrium of the last collision, feq(rho, u), rebuilt on the host."""
+        return _FeqView(self)
+
+    @property
+    def fpc(self):
+        raise AttributeError("post-collision populations are not materialised: the collide-stream "
+                             "kernel writes them straight into the streamed slots")
+
+
+class _FeqView:
+    def __init__(self, lattice):
+        self._lat = lattice
+
+    def to_numpy(self):
+        L = self._lat
+        if L._lag is not None:
+            lag = L._lag.view(5, L.Nx, L.Ny, L.Nz).cpu().numpy().astype(np.float64)
+            rho, u = lag[0], np.stack([lag[1], lag[2], lag[3]], axis=-1)
+        else:
+            rho, u = L.rho.to_numpy().astype(np.float64), L.vel.to_numpy().astype(np.float64)
+        cu = u @ L.c.T
+        usq = (u * u).sum(-1, keepdims=True)
+        return L.w * rho[..., None] * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq)

@@ -1,0 +1,241 @@
+"""ORACLE -- test infrastructure only (imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package).
+
+ctypes wrapper of liblbdem_oracle.so, the C restatement of the reference's algorithms, plus
+small drivers that sequence the restated kernels the way the reference's solver classes do.
+Arrays follow the reference's layout: lattice fields are [i, j, k] (k fastest), vectors AoS.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbdem_oracle.so")
+
+FLUID, OBSTACLE, FREE_SLIP, VEL_LADD, VEL_INLET_LADD = 1, 8, 16, 32, 262144
+SKIP_BASIC = OBSTACLE | VEL_LADD | FREE_SLIP | VEL_INLET_LADD   # lbm_solver3d.py:183
+SKIP_FORCED = OBSTACLE | VEL_LADD | FREE_SLIP                   # lbm_foecelover.py:45, psclattice3d.py:204
+
+C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+              [1, 1, 0], [-1, -1, 0], [1, -1, 0], [-1, 1, 0], [1, 0, 1], [-1, 0, -1], [1, 0, -1],
+              [-1, 0, 1], [0, 1, 1], [0, -1, -1], [0, 1, -1], [0, -1, 1]], dtype=np.float64)
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+_lib = None
+_dp = ctypes.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build_oracle  # noqa: F401  (only when run as package)
+        _lib = ctypes.CDLL(LIB_PATH)
+        for sfx in ("_f64", "_f32"):
+            getattr(_lib, "lbo_threedelta" + sfx).restype = ctypes.c_double
+            getattr(_lib, "lbo_threedelta" + sfx).argtypes = [ctypes.c_double]
+            getattr(_lib, "lbo_tenneti_cd" + sfx).restype = ctypes.c_double
+            getattr(_lib, "lbo_tenneti_cd" + sfx).argtypes = [ctypes.c_double] * 3
+            getattr(_lib, "lbo_kernel_mirror" + sfx).restype = ctypes.c_double
+            getattr(_lib, "lbo_kernel_mirror" + sfx).argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_int] * 3
+    return _lib
+
+
+def P(a):
+    """Pointer of a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data_as(_dp)
+
+
+def _fn(name, real):
+    return getattr(lib(), name + ("_f64" if real == np.float64 else "_f32"))
+
+
+def _d(x):
+    return ctypes.c_double(float(x))
+
+
+class OracleLattice:
+    """Reference-layout lattice state and its two-pass step (collide -> fpc, stream -> f).
+
+    kind: "basic" (lagged Smagorinsky, BasicLattice3D), "forced" (ForcedLattice3D),
+    "psc" (PSCLattice3D weighted collision), "eqibm" (EqIMBlattice3D), "vans".
+    """
+
+    def __init__(self, nx, ny, nz, omega, kind="basic", cs=0.1, F=None, real=np.float64):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.n = nx * ny * nz
+        self.omega0 = omega
+        self.kind = kind
+        self.cs = cs
+        self.real = real
+        self.F = None if F is None else np.asarray(F, dtype=real)
+        sh = (nx, ny, nz)
+        self.CT = np.full(sh, FLUID, dtype=np.int32)
+        self.f = np.zeros(sh + (19,), dtype=real)
+        self.feq = np.zeros(sh + (19,), dtype=real)
+        self.fpc = np.zeros(sh + (19,), dtype=real)
+        self.rho = np.ones(sh, dtype=real)
+        self.vel = np.zeros(sh + (3,), dtype=real)
+        self.omega = np.full(sh, omega, dtype=real)
+        self.rate = np.zeros(sh, dtype=real)
+        # coupling fields
+        self.volfrac = np.zeros(sh, dtype=real)
+        self.weight = np.zeros(sh, dtype=real)
+        self.velsolid = np.zeros(sh + (3,), dtype=real)
+        self.hydroforce = np.zeros(sh + (3,), dtype=real)
+        self.id = np.full(sh, -1, dtype=np.int32)
+
+    @property
+    def skip(self):
+        return SKIP_BASIC if self.kind in ("basic", "eqibm") else SKIP_FORCED
+
+    def _g(self):
+        return (ctypes.c_int(self.nx), ctypes.c_int(self.ny), ctypes.c_int(self.nz))
+
+    def initialize(self):
+        _fn("lbo_initialize", self.real)(*self._g(), P(self.CT), ctypes.c_int32(self.skip), P(self.rho),
+                                         P(self.vel), P(self.feq), P(self.f))
+
+    def collide(self):
+        R = ctypes.c_double
+        if self.kind in ("basic", "forced"):
+            smag = 1 if self.kind == "basic" else 0
+            _fn("lbo_collide_basic", self.real)(*self._g(), P(self.CT), ctypes.c_int32(self.skip),
+                                                ctypes.c_int32(smag), _d(self.omega0), _d(self.cs),
+                                                P(self.F), P(self.f), P(self.feq), P(self.rho), P(self.vel),
+                                                P(self.omega), P(self.rate), P(self.fpc))
+        elif self.kind in ("psc", "eqibm"):
+            smag = 1 if self.kind == "eqibm" else 0
+            _fn("lbo_collide_weighted", self.real)(*self._g(), P(self.CT), ctypes.c_int32(self.skip),
+                                                   ctypes.c_int32(smag), _d(self.omega0), _d(self.cs),
+                                                   P(self.F), P(self.f), P(self.feq), P(self.rho), P(self.vel),
+                                                   P(self.omega), P(self.rate), P(self.volfrac), P(self.weight),
+                                                   P(self.velsolid),
+                                                   P(self.hydroforce) if self.kind == "psc" else None, P(self.fpc))
+        elif self.kind == "vans":
+            _fn("lbo_vans_collide", self.real)(*self._g(), P(self.CT), R(self.omega0), P(self.f), P(self.feq),
+                                               P(self.rho), P(self.vel), P(self.omega), P(self.volfrac),
+                                               P(self.hydroforce), P(self.fpc))
+        else:
+            raise ValueError(self.kind)
+
+    def stream(self):
+        vans = 1 if self.kind == "vans" else 0
+        _fn("lbo_stream", self.real)(*self._g(), P(self.CT), ctypes.c_int32(self.skip), P(self.fpc), P(self.f),
+                                     P(self.rho), P(self.vel), ctypes.c_int32(vans),
+                                     P(self.volfrac) if vans else None, P(self.hydroforce) if vans else None)
+
+    def step(self, n=1):
+        for _ in range(n):
+            self.collide()
+            self.stream()
+
+    # conservation sums over collided cells, from the post-stream populations
+    def sums(self):
+        act = (self.CT & self.skip) == 0
+        f = self.f[act].astype(np.float64)
+        rho = f.sum(-1)
+        j = f @ C
+        return rho.sum(), j.sum(0), 0.5 * ((j * j).sum(-1) / rho).sum()
+
+
+# ---------------------------------------------------------------------------------------
+# coupling restatements
+# ---------------------------------------------------------------------------------------
+def psc_map(L, grains, dmin, dx, dt):
+    """PSCLattice3D.grains2lattice (psclattice3d.py:84-182); grains (n, 10)."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    _fn("lbo_psc_map", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                               P(L.omega), P(L.id), P(L.volfrac), P(L.weight), P(L.velsolid))
+
+
+def psc_forces(L, grains, dmin, dx, dt, rho_phys):
+    """PSCLattice3D.lattice2grains (psclattice3d.py:316-358) -> (n, 6) [F, T]."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    out = np.zeros((len(g), 6))
+    _fn("lbo_psc_forces", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                                  _d(rho_phys), P(L.id), P(L.volfrac), P(L.hydroforce), P(out))
+    return out
+
+
+def threedelta(r):
+    return lib().lbo_threedelta_f64(float(r))
+
+
+def kernel_mirror(nx, ny, nz, xc, yc, zc, i, j, k):
+    return lib().lbo_kernel_mirror_f64(nx, ny, nz, xc, yc, zc, i, j, k)
+
+
+def tenneti_cd(Re, svf, lim=1e-9):
+    return lib().lbo_tenneti_cd_f64(float(Re), float(svf), float(lim))
+
+
+def ibm_spread(L, grains, dmin, dx, dt):
+    """EqIMBlattice3D.grains2lattice (eqlattice.py:104-194); grains (n, 7) [x, v, R]."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    velsum = np.zeros_like(L.velsolid)
+    wsum = np.zeros_like(L.volfrac)
+    _fn("lbo_ibm_spread", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                                  P(L.volfrac), P(L.velsolid), P(velsum), P(wsum))
+    return velsum, wsum
+
+
+def ibm_weight(L, dx, dt, rho_phys):
+    """EqIMBlattice3D.compute_weight (eqlattice.py:200-238)."""
+    _fn("lbo_ibm_weight", L.real)(*L._g(), P(L.CT), _d(dx), _d(dt), _d(rho_phys), _d(L.omega0), P(L.volfrac),
+                                  P(L.velsolid), P(L.vel), P(L.weight))
+
+
+def ibm_drag(L, grains, dmin, dx, dt, rho_phys):
+    """EqIMBlattice3D.lattice2grains (eqlattice.py:309-386) -> (n, 3) force_fluid."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    out = np.zeros((len(g), 3))
+    _fn("lbo_ibm_drag", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                                _d(rho_phys), _d(L.omega0), P(L.vel), P(L.volfrac), P(out))
+    return out
+
+
+def vans_spread(L, grains, dmin, dx, dt, rho_phys, with_force=True):
+    """Unresolvedlattice3D.force2lattice / grains2lattice; grains (n, 7) [x, R, F_fluid]."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    _fn("lbo_vans_spread", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                                   _d(rho_phys), ctypes.c_int32(1 if with_force else 0), P(L.volfrac),
+                                   P(L.hydroforce))
+
+
+def vans_drag(L, grains, dmin, dx, dt, rho_phys):
+    """Unresolvedlattice3D.compute_hydrodynamic_force; grains (n, 7) [x, v, R] -> (n, 3)."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    out = np.zeros((len(g), 3))
+    _fn("lbo_vans_drag", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                                 _d(rho_phys), _d(L.omega0), P(L.volfrac), P(L.vel), P(out))
+    return out
+
+
+def cavity(L, lid_u, lid_axis=2):
+    """Lid-driven cavity cell types (lid_driven_cavity.py:31-47 with the lid on the +lid_axis face)."""
+    CT = L.CT
+    CT[...] = FLUID
+    for ax in range(3):
+        sl_lo = [slice(None)] * 3
+        sl_hi = [slice(None)] * 3
+        sl_lo[ax] = 0
+        sl_hi[ax] = -1
+        CT[tuple(sl_lo)] = OBSTACLE
+        if ax != lid_axis:
+            CT[tuple(sl_hi)] = OBSTACLE
+    lid = [slice(1, -1)] * 3
+    lid[lid_axis] = -1
+    CT[tuple(lid)] = VEL_LADD
+    L.vel[tuple(lid)] = np.asarray(lid_u, dtype=L.real)
+    return CT
