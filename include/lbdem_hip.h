@@ -72,6 +72,9 @@ typedef struct lbdem_lattice {
   double cs_smag;      /* Smagorinsky constant */
   double rho0;         /* shift density of the stored populations */
   double force[3];     /* body force in lattice units: adds rho*w_q*(c_q.F)/cs^2 */
+  int32_t kernel;      /* collide-stream variant: 0 automatic, 1 one cell per thread,
+                          2 row-tiled (one workgroup per (i, j) row, aligned accesses) */
+  int32_t reserved;
 } lbdem_lattice;
 
 /* Per-step device pointers.  Unused members may be NULL. */
@@ -79,7 +82,8 @@ typedef struct lbdem_step_args {
   void *f;                     /* 19*ncell populations, storage type */
   const uint8_t *flags;        /* ncell flag bytes */
   const float *vel;            /* 3*ncell (AoS): wall velocity read at MOVING cells */
-  float *lag;                  /* 5*ncell (SoA rho,ux,uy,uz,omega) for SMAG_LAGGED */
+  void *lag;                   /* 5*ncell (SoA rho,ux,uy,uz,omega) for SMAG_LAGGED; float,
+                                  double with LBDEM_STORE_F64 */
   /* PSC */
   const uint32_t *psc_key;     /* ncell packed (full, count, grain) keys, 0 = no grain */
   const float *psc_grain;      /* 12*ngrain: xc,yc,zc, v_lu xyz, omega*dt xyz, unused x3 */
@@ -142,6 +146,117 @@ int lbdem_lbm_write_f(const lbdem_lattice *lat, const lbdem_step_args *a, const 
 int64_t lbdem_reduce_work_doubles(const lbdem_lattice *lat);
 int lbdem_lbm_reduce(const lbdem_lattice *lat, const lbdem_step_args *a, double *work,
                      double *out, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* grain / particle state shared by the couplings and the DEM (device pointers, FP64)     */
+/* ------------------------------------------------------------------------------------ */
+typedef struct lbdem_grains {
+  int32_t n;                 /* number of grains */
+  const double *position;    /* 3n, physical units */
+  const double *velocity;    /* 3n */
+  const double *omega;       /* 3n angular velocity */
+  const double *radius;      /* n */
+  double *force_fluid;       /* 3n, written by the couplings */
+  double *moment_fluid;      /* 3n */
+} lbdem_grains;
+
+/* lattice <-> physical units (UnitConverter, src/lbm3d/lbmutils.py:14-63) and the DEM
+ * domain minimum: a grain at x sits at lattice coordinate (x - dmin)/dx + 1/2. */
+typedef struct lbdem_units {
+  double dx, dt, rho;
+  double dmin[3];
+} lbdem_units;
+
+/* PSC grain -> lattice mapping.  Replaces PSCLattice3D.grains2lattice with compute_weight and
+ * compute_solid_vel (src/lbdem/psclattice3d.py:84-182): writes per-cell packed keys
+ * (full bit 30, sub-cell count bits 29..23, 0x7FFFFF - grain bits 22..0; 0 = uncovered),
+ * a 12-float per-grain snapshot (centre in lattice units, v*dt/dx, omega*dt) read by the
+ * collision, and each grain's 6-int box.  clear_previous zeroes the keys inside the boxes
+ * of the previous call first.  Grain-centric; geometry in FP64. */
+int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                  const lbdem_units *u, uint32_t *key, float *snap, int32_t *bbox,
+                  int32_t clear_previous, void *stream);
+
+/* Per-grain hydrodynamic force and torque (physical units, FP64, fixed-order reduction) from
+ * the per-cell force the PSC collision wrote.  Replaces PSCLattice3D.lattice2grains
+ * (src/lbdem/psclattice3d.py:316-358). */
+int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u,
+                     const uint32_t *key, const float *cell_force, void *stream);
+
+/* id / volfrac / weight / velsolid fields (psclattice3d.py:45-50) decoded from the keys. */
+int lbdem_psc_decode(const lbdem_lattice *lat, const uint32_t *key, const float *snap, int32_t *id,
+                     float *volfrac, float *weight, float *velsolid, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* DEM (src/dem3d)                                                                        */
+/* ------------------------------------------------------------------------------------ */
+/* material row layout: density, E, nu, friction, restitution, k_n, k_t, damping_n, damping_t */
+typedef struct lbdem_dem_params {
+  int32_t n;            /* particles */
+  int32_t K;            /* history slots per particle (max_coordinate_number) */
+  int32_t model;        /* 0 Hertz-Mindlin, 1 linear spring-dashpot */
+  int32_t T;            /* hash table size, power of two (BPCD.create, bpcd.py:48-65) */
+  int32_t probe27;      /* 0: 8-cell octant probe (reference); 1: 27-cell probe */
+  int32_t periodic[3];  /* periodic axes */
+  double dt;            /* DEM time step */
+  double g[3];          /* gravity */
+  double dmin[3], dmax[3]; /* DEMSolverConfig.domain bounds */
+  double cell;          /* hash cell size (reference: 4 r_max) */
+  double mat[2][9];     /* [0] particle, [1] wall material */
+  double wall_n[6][3];  /* outward wall normals (DEMSolverConfig._setup_walls) */
+  double wall_d[6];     /* n.x = d */
+  int32_t wall_type[6]; /* 0 wall, 1 periodic (skipped) */
+} lbdem_dem_params;
+
+/* Particle and contact state, device pointers (FP64 unless noted).  The Grain / Contact
+ * structs of the reference (src/dem3d/dateclass/grain.py:23-52, contact.py:20-33) as
+ * structure-of-arrays: vectors AoS per particle (3 or 4 or 9 doubles). */
+typedef struct lbdem_dem_state {
+  double *mass, *radius;             /* n */
+  double *pos, *vel, *acc, *force;   /* 3n */
+  double *quat;                      /* 4n (w, x, y, z) */
+  double *omega, *omega_dot, *moment;/* 3n */
+  double *inertia;                   /* 9n */
+  double *force_fluid, *moment_fluid;/* 3n, added every substep */
+  const int32_t *freeze, *fixvel;    /* n, may be NULL */
+  int32_t *cf_j, *cf_active;         /* n*K history slots of particle i: partner j > i */
+  int32_t *cfn;                      /* n slots in use */
+  double *cf_pos, *cf_force, *cf_shear; /* 3*n*K */
+  int32_t *wcf_active;               /* 6n wall contacts */
+  double *wcf_pos, *wcf_force, *wcf_shear; /* 18n */
+  int32_t *status;                   /* 8 counters: [0] slot overflow, [1] dt above the
+                                        contact-law critical step, [2] particle overlap > 5%,
+                                        [3] wall overlap > 5%, [4] candidate-list overflow
+                                        (required length) */
+} lbdem_dem_state;
+
+/* Caller-provided DEM scratch.  count/current must be zero on first use; the library leaves
+ * them zero.  npairs must be zero before lbdem_dem_broadphase; lbdem_dem_run resets it. */
+typedef struct lbdem_dem_work {
+  int32_t *count, *offset, *current; /* T each */
+  int32_t *tile_sum;                 /* ceil(T / 1024) */
+  int32_t *hash, *pid;               /* n each */
+  int32_t *probe;                    /* 27n (small-system path) */
+  int32_t *pairs;                    /* 2 * pair_cap candidate pairs (i < j) */
+  int64_t pair_cap;
+  unsigned long long *npairs;        /* 1: candidate count (may exceed pair_cap) */
+  int32_t *need_new;                 /* pair_cap */
+} lbdem_dem_work;
+
+/* Candidate pairs of the Morton-hash broad phase: BPCD.detect_collision with the count,
+ * Blelloch scan, scatter and 8-octant probe (src/dem3d/bpcd/bpcd.py:67-152,
+ * src/dem3d/bpcd/prefixsum.py:62-84).  Pairs (i, j), i < j, land unordered in w->pairs;
+ * w->npairs receives their number (also when it exceeds pair_cap). */
+int lbdem_dem_broadphase(const lbdem_dem_params *P, const double *pos, const lbdem_dem_work *w,
+                         void *stream);
+
+/* nsub DEM substeps, each DEMSolver.run_simulation (src/dem3d/demsolver/demsolver.py:848-862):
+ * clear_state, contact (broad phase + resolve :638-740), resolve_wall (:809-838),
+ * apply_body_force (:437-448), update (:514-636), late_clear_state (:431-436).
+ * path: 0 automatic, 1 single-workgroup path (n <= 2048, one launch for all substeps),
+ * 2 general grid-wide path. */
+int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbdem_dem_work *w,
+                  int32_t nsub, int32_t path, void *stream);
 
 #ifdef __cplusplus
 }

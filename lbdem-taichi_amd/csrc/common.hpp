@@ -89,7 +89,7 @@ struct KParams {
   void *f;
   const uint8_t *flags;
   const float *vel;
-  float *lag;
+  void *lag;  // float, or double with FP64 storage
   const uint32_t *psc_key;
   const float *psc_grain;
   float *psc_force;

@@ -1,0 +1,788 @@
+// Discrete-element method on gfx950: hash broad phase, contact resolution with per-particle
+// history slots, Hertz-Mindlin / linear spring-dashpot laws, planar walls, integration.
+//
+// Two execution paths share the per-contact device code:
+//  - the general path: about ten grid-wide kernels per DEM substep (broad phase with the
+//    reference's Morton hash and counting sort, two-phase contact resolution, walls,
+//    integration, slot compaction);
+//  - the small-system path (n <= kSmallN): one workgroup runs many substeps in a single
+//    launch, with barriers between phases, and finds candidate pairs by comparing each
+//    particle's cell hash with the probe-cell hashes of the other, which yields exactly the
+//    set the hash-table probe yields.
+//
+// FP64 throughout.  The float -> cell arithmetic is compiled without contraction
+// (-ffp-contract=off) so cell keys match the CPU restatement bit for bit.
+#include "common.hpp"
+
+namespace lbdem {
+
+constexpr int kSmallN = 2048;
+
+struct DemP {  // lbdem_dem_params, copied into the kernel argument block
+  lbdem_dem_params p;
+};
+
+// --------------------------------------------------------------------------------------
+// broad phase (src/dem3d/bpcd/bpcd.py)
+// --------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread10(int32_t v) {
+  uint32_t x = (uint32_t)v & 0x3ffu;
+  x = (x | x << 16) & 0x30000ffu;
+  x = (x | x << 8) & 0x300f00fu;
+  x = (x | x << 4) & 0x30c30c3u;
+  x = (x | x << 2) & 0x9249249u;
+  return x;
+}
+// morton3d32 % T  (bpcd.py:154-189)
+__device__ __forceinline__ int32_t hash3(int32_t x, int32_t y, int32_t z, int32_t T) {
+  return (int32_t)((spread10(x) | (spread10(y) << 1) | (spread10(z) << 2)) & (uint32_t)(T - 1));
+}
+// cell = floor((x - min) / cs)  (bpcd.py:191-194)
+__device__ __forceinline__ void cell_of(const lbdem_dem_params &P, const double *x, int32_t c[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) c[a] = (int32_t)floor((x[a] - P.dmin[a]) / P.cell);
+}
+
+// hashes of the probed cells of particle i: the octant's 8 cells (bpcd.py:114-152) or 27;
+// duplicates (hash aliasing) are replaced by -1 so every bucket is visited once.
+__device__ __forceinline__ int probe_hashes(const lbdem_dem_params &P, const double *x, int32_t *h) {
+  int32_t c[3];
+  cell_of(P, x, c);
+  int nc = 0;
+  if (!P.probe27) {
+    int32_t d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double center = ((double)c[a] + 0.5) * P.cell + P.dmin[a];
+      d[a] = (x[a] > center) ? 1 : -1;
+    }
+    const int32_t o[8][3] = {{0, 0, 0},       {d[0], 0, 0},    {0, d[1], 0},    {0, 0, d[2]},
+                             {0, d[1], d[2]}, {d[0], 0, d[2]}, {d[0], d[1], 0}, {d[0], d[1], d[2]}};
+    for (int k = 0; k < 8; ++k) h[nc++] = hash3(c[0] + o[k][0], c[1] + o[k][1], c[2] + o[k][2], P.T);
+  } else {
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) h[nc++] = hash3(c[0] + dx, c[1] + dy, c[2] + dz, P.T);
+  }
+  for (int k = 1; k < nc; ++k)
+    for (int e = 0; e < k; ++e)
+      if (h[e] == h[k]) {
+        h[k] = -1;
+        break;
+      }
+  return nc;
+}
+
+__global__ __launch_bounds__(256) void k_bp_count(const lbdem_dem_params P, const double *pos, int32_t *hash,
+                                                  int32_t *count, double *force, double *moment) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  int32_t c[3];
+  cell_of(P, pos + 3 * i, c);
+  const int32_t h = hash3(c[0], c[1], c[2], P.T);
+  hash[i] = h;
+  atomicAdd(count + h, 1);
+  if (force) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) force[3 * i + a] = moment[3 * i + a] = 0.0;  // clear_state
+  }
+}
+
+// exclusive scan of count[T] -> offset[T]: per-1024 tiles, tile sums, then add
+__global__ __launch_bounds__(256) void k_scan_tiles(const int32_t *in, int32_t *out, int32_t *tile_sum, int n) {
+  __shared__ int32_t sh[256];
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  int32_t v[4], s = 0;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    v[m] = (base + m < n) ? in[base + m] : 0;
+    s += v[m];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int32_t t = (threadIdx.x >= (unsigned)o) ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int32_t run = sh[threadIdx.x] - s;  // exclusive prefix of this thread
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    if (base + m < n) out[base + m] = run;
+    run += v[m];
+  }
+  if (threadIdx.x == 255) tile_sum[blockIdx.x] = sh[255];
+}
+__global__ __launch_bounds__(1024) void k_scan_sums(int32_t *tile_sum, int ntiles) {
+  __shared__ int32_t sh[1024];
+  int32_t carry = 0;
+  for (int base = 0; base < ntiles; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int32_t v = (i < ntiles) ? tile_sum[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int32_t t = (threadIdx.x >= (unsigned)o) ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < ntiles) tile_sum[i] = carry + sh[threadIdx.x] - v;
+    const int32_t tot = sh[1023];
+    __syncthreads();
+    carry += tot;
+  }
+}
+__global__ __launch_bounds__(256) void k_scan_add(int32_t *out, const int32_t *tile_sum, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] += tile_sum[i >> 10];
+}
+
+__global__ __launch_bounds__(256) void k_bp_scatter(const lbdem_dem_params P, const int32_t *hash,
+                                                    const int32_t *offset, int32_t *current, int32_t *pid) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  const int32_t h = hash[i];
+  pid[offset[h] + atomicAdd(current + h, 1)] = i;
+}
+
+// probe: candidate pairs (i, pid > i); one atomic per wavefront reserves the output range
+__global__ __launch_bounds__(256) void k_bp_probe(const lbdem_dem_params P, const double *pos,
+                                                  const int32_t *count, const int32_t *offset, const int32_t *pid,
+                                                  int32_t *pairs, int64_t cap, unsigned long long *npairs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int32_t h[27];
+  int nc = 0;
+  int mine = 0;
+  if (i < P.n) {
+    nc = probe_hashes(P, pos + 3 * i, h);
+    for (int k = 0; k < nc; ++k) {
+      if (h[k] < 0) continue;
+      const int32_t b = offset[h[k]], e = b + count[h[k]];
+      for (int idx = b; idx < e; ++idx) mine += (pid[idx] > i) ? 1 : 0;
+    }
+  }
+  // wave-level exclusive scan of `mine`
+  const int lane = threadIdx.x & 63;
+  int incl = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  const int total = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if (lane == 63 && total > 0) base = atomicAdd(npairs, (unsigned long long)total);
+  base = __shfl(base, 63, 64);
+  unsigned long long w = base + (unsigned long long)(incl - mine);
+  if (i < P.n && mine > 0) {
+    for (int k = 0; k < nc; ++k) {
+      if (h[k] < 0) continue;
+      const int32_t b = offset[h[k]], e = b + count[h[k]];
+      for (int idx = b; idx < e; ++idx) {
+        const int32_t p = pid[idx];
+        if (p > i) {
+          if (w < (unsigned long long)cap) {
+            pairs[2 * w] = i;
+            pairs[2 * w + 1] = p;
+          }
+          ++w;
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bp_reset(const lbdem_dem_params P, const int32_t *hash, int32_t *count,
+                                                  int32_t *current) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  count[hash[i]] = 0;
+  current[hash[i]] = 0;
+}
+
+// --------------------------------------------------------------------------------------
+// contact mechanics
+// --------------------------------------------------------------------------------------
+enum { MAT_RHO, MAT_E, MAT_NU, MAT_MU, MAT_REST, MAT_KN, MAT_KT, MAT_DPN, MAT_DPS };
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double norm3(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+__device__ __forceinline__ void matvec(const double *M, const double *v, double *o) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) o[r] = M[3 * r] * v[0] + M[3 * r + 1] * v[1] + M[3 * r + 2] * v[2];
+}
+// 3x3 inverse by the adjugate (ti.Matrix.inverse for n = 3)
+__device__ __forceinline__ void inv3(const double *m, double *o) {
+  const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                     m[2] * (m[3] * m[7] - m[4] * m[6]);
+  const double id = 1.0 / det;
+  o[0] = (m[4] * m[8] - m[5] * m[7]) * id;
+  o[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+  o[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+  o[3] = (m[5] * m[6] - m[3] * m[8]) * id;
+  o[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+  o[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+  o[6] = (m[3] * m[7] - m[4] * m[6]) * id;
+  o[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+  o[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+}
+// rotation taking the unit vector a to +x (Rodrigues; demsolver.py:686-701)
+__device__ __forceinline__ void rot_to_x(const double *a, double *R) {
+  const double v[3] = {a[1] * 0.0 - a[2] * 0.0, a[2] * 1.0 - a[0] * 0.0, a[0] * 0.0 - a[1] * 1.0};
+  const double s = norm3(v), c = a[0] * 1.0 + a[1] * 0.0 + a[2] * 0.0;
+  if (s < 1e-12) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = 0.0;
+    if (c > 0.0) R[0] = R[4] = R[8] = 1.0;
+    else {
+      R[0] = -1.0;
+      R[4] = 1.0;
+      R[8] = -1.0;
+    }
+    return;
+  }
+  const double vx[9] = {0.0, -v[2], v[1], v[2], 0.0, -v[0], -v[1], v[0], 0.0};
+  const double k = (1.0 - c) / (s * s);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double vv = vx[3 * r] * vx[q] + vx[3 * r + 1] * vx[3 + q] + vx[3 * r + 2] * vx[6 + q];
+      R[3 * r + q] = ((r == q) ? 1.0 : 0.0) + vx[3 * r + q] + k * vv;
+    }
+}
+
+__device__ __forceinline__ void periodic_delta(const lbdem_dem_params &P, const double *xi, const double *xj,
+                                               double *d) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    d[a] = xj[a] - xi[a];
+    if (P.periodic[a]) {
+      const double L = P.dmax[a] - P.dmin[a];
+      if (d[a] > 0.5 * L) d[a] -= L;
+      else if (d[a] < -0.5 * L) d[a] += L;
+    }
+  }
+}
+
+// contact law in the local frame (hertz.py:45-153, linear.py:43-152)
+__device__ void contact_law(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j, bool wall,
+                            double delta_n, const double *vc, double *shear, double *F) {
+  const double *mi = P.mat[0], *mj = P.mat[wall ? 1 : 0];
+  const double dt = P.dt;
+  const double gap = -1.0 * delta_n;
+  double kn, kt, gn, gt, mu;
+  const double pi = 3.14159265358979323846;
+  if (P.model == 0) {
+    const double Ys = 1.0 / ((1.0 - mi[MAT_NU] * mi[MAT_NU]) / mi[MAT_E] + (1.0 - mj[MAT_NU] * mj[MAT_NU]) / mj[MAT_E]);
+    const double Gs = 1.0 / (2.0 * (2.0 - mi[MAT_NU]) * (1.0 + mi[MAT_NU]) / mi[MAT_E] +
+                             2.0 * (2.0 - mj[MAT_NU]) * (1.0 + mj[MAT_NU]) / mj[MAT_E]);
+    double Rs, ms, e;
+    if (wall) {
+      Rs = S.radius[i];
+      ms = S.mass[i];
+      e = mj[MAT_REST];
+      mu = mj[MAT_MU];
+    } else {
+      Rs = 1.0 / (1.0 / S.radius[i] + 1.0 / S.radius[j]);
+      const double ri = S.radius[i];
+      const double rho = S.mass[i] * 3 / (4 * pi * ri * ri * ri);
+      const double tmin = pi * Rs * sqrt(rho / Gs) / (0.1631 * mi[MAT_NU] + 0.8766);  // hertz.py:156-162
+      if (dt > tmin) atomicAdd(S.status + 1, 1);
+      ms = 1.0 / (1.0 / S.mass[i] + 1.0 / S.mass[j]);
+      e = 0.5 * (mi[MAT_REST] + mj[MAT_REST]);
+      mu = 0.5 * (mi[MAT_MU] + mj[MAT_MU]);
+    }
+    const double le = log(e);
+    const double beta = le / sqrt(le * le + pi * pi);
+    const double Sn = 2.0 * Ys * sqrt(Rs * delta_n);
+    const double St = 8.0 * Gs * sqrt(Rs * delta_n);
+    kn = 4.0 / 3.0 * Ys * sqrt(Rs * delta_n);
+    gn = -2.0 * beta * sqrt(5.0 / 6.0 * Sn * ms);
+    kt = 8.0 * Gs * sqrt(Rs * delta_n);
+    gt = -2.0 * beta * sqrt(5.0 / 6.0 * St * ms);
+  } else {
+    double ms, dpn, dps;
+    if (wall) {
+      kn = mi[MAT_KN];
+      kt = mi[MAT_KT];
+      dpn = mi[MAT_DPN];
+      dps = mi[MAT_DPS];
+      ms = S.mass[i];
+      mu = mj[MAT_MU];
+    } else {
+      kn = mi[MAT_KN] * mj[MAT_KN] / (mi[MAT_KN] + mj[MAT_KN]);
+      kt = mi[MAT_KT] * mj[MAT_KT] / (mi[MAT_KT] + mj[MAT_KT]);
+      dpn = 0.5 * (mi[MAT_DPN] + mj[MAT_DPN]);
+      dps = 0.5 * (mi[MAT_DPS] + mj[MAT_DPS]);
+      if (dt > sqrt(S.mass[i] / mi[MAT_KN])) atomicAdd(S.status + 1, 1);  // linear.py:155-160
+      ms = 1.0 / (1.0 / S.mass[i] + 1.0 / S.mass[j]);
+      mu = 0.5 * (mi[MAT_MU] + mj[MAT_MU]);
+    }
+    gn = 2. * dpn * sqrt(kn * ms);
+    gt = 2. * dps * sqrt(kn * ms);
+  }
+  shear[1] += vc[1] * dt;
+  shear[2] += vc[2] * dt;
+  F[0] = -kn * gap - gn * vc[0];
+  const double tr[3] = {-kt * shear[0], -kt * shear[1], -kt * shear[2]};
+  const double tl = norm3(tr);
+  if (tl >= mu * F[0] && tl > 0.0) {
+    const double ratio = mu * F[0] / tl;
+    F[1] = tr[1] * ratio;
+    F[2] = tr[2] * ratio;
+    shear[1] = -F[1] / kt;
+    shear[2] = -F[2] / kt;
+  } else {
+    F[1] = tr[1] - gt * vc[1];
+    F[2] = tr[2] - gt * vc[2];
+  }
+}
+
+// evaluate an active particle-particle contact held in slot `off` (demsolver.py:683-740)
+__device__ void eval_pair(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j, int off, const double *d,
+                          double dist, double gap) {
+  double a[3] = {d[0] / dist, d[1] / dist, d[2] / dist}, R[9], Ri[9];
+  rot_to_x(a, R);
+  const double delta_n = -gap;
+  const double Rmin = fmin(S.radius[i], S.radius[j]);
+  if (delta_n > 0.05 * Rmin) atomicAdd(S.status + 2, 1);
+  const double *xi = S.pos + 3 * i;
+  double cp[3], ri[3], rj[3], vci[3], vcj[3], dv[3], vc[3], F[3], sh[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) cp[k] = xi[k] + a[k] * (S.radius[i] - delta_n);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    S.cf_pos[3 * off + k] = cp[k];
+    ri[k] = cp[k] - xi[k];
+    rj[k] = cp[k] - (xi[k] + d[k]);
+    sh[k] = S.cf_shear[3 * off + k];
+  }
+  cross3(S.omega + 3 * i, ri, vci);
+  cross3(S.omega + 3 * j, rj, vcj);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) dv[k] = (vcj[k] + S.vel[3 * j + k]) - (vci[k] + S.vel[3 * i + k]);
+  matvec(R, dv, vc);
+  contact_law(P, S, i, j, false, delta_n, vc, sh, F);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    S.cf_shear[3 * off + k] = sh[k];
+    S.cf_force[3 * off + k] = F[k];
+  }
+  inv3(R, Ri);
+  const double mF[3] = {-F[0], -F[1], -F[2]};
+  double Fi[3], Fj[3], Mi[3], Mj[3];
+  matvec(Ri, mF, Fi);
+  matvec(Ri, F, Fj);
+  cross3(ri, Fi, Mi);
+  cross3(rj, Fj, Mj);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    unsafeAtomicAdd(S.force + 3 * i + k, Fi[k]);
+    unsafeAtomicAdd(S.force + 3 * j + k, Fj[k]);
+    unsafeAtomicAdd(S.moment + 3 * i + k, Mi[k]);
+    unsafeAtomicAdd(S.moment + 3 * j + k, Mj[k]);
+  }
+}
+
+// Phase 1 of resolve (demsolver.py:638-668): pairs with an active history slot are evaluated
+// (gap < 0) or released (gap >= 0).  Returns 1 when the pair needs a new slot.  Only reads
+// slots of other pairs, so it can run for all pairs concurrently.
+__device__ int resolve_existing(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j) {
+  const int K = P.K;
+  const int ncf = min(S.cfn[i], K);
+  int off = -1;
+  for (int s = 0; s < ncf; ++s)
+    if (S.cf_j[i * K + s] == j && S.cf_active[i * K + s]) {
+      off = i * K + s;
+      break;
+    }
+  double d[3];
+  periodic_delta(P, S.pos + 3 * i, S.pos + 3 * j, d);
+  const double dist = norm3(d);
+  const double gap = dist - S.radius[i] - S.radius[j];
+  if (off >= 0) {
+    if (gap < 0) eval_pair(P, S, i, j, off, d, dist, gap);
+    else S.cf_active[off] = 0;
+    return 0;
+  }
+  return gap < 0 ? 1 : 0;
+}
+
+// Phase 2: a new contact takes the next slot of i (append_contact_offset, demsolver.py:352-359)
+__device__ void resolve_new(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j) {
+  const int K = P.K;
+  const int slot = atomicAdd(S.cfn + i, 1);
+  if (slot >= K) {  // the reference prints and writes cf[-1]; here: counted, no write
+    atomicAdd(S.status + 0, 1);
+    return;
+  }
+  const int off = i * K + slot;
+  S.cf_j[off] = j;
+  S.cf_active[off] = 1;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) S.cf_force[3 * off + k] = S.cf_shear[3 * off + k] = 0.0;
+  double d[3];
+  periodic_delta(P, S.pos + 3 * i, S.pos + 3 * j, d);
+  const double dist = norm3(d);
+  eval_pair(P, S, i, j, off, d, dist, dist - S.radius[i] - S.radius[j]);
+}
+
+// resolve_wall + evaluate_wall (demsolver.py:742-838) for particle i; forces added in place
+__device__ void resolve_walls(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
+  for (int w = 0; w < 6; ++w) {
+    if (P.wall_type[w] == 1) continue;
+    const double *nrm = P.wall_n[w];
+    const double *x = S.pos + 3 * i;
+    const double pd = x[0] * nrm[0] + x[1] * nrm[1] + x[2] * nrm[2] - P.wall_d[w];
+    const int c = 6 * i + w;
+    if (S.wcf_active[c]) {
+      if (-pd > S.radius[i]) {
+        S.wcf_active[c] = 0;
+        continue;
+      }
+    } else {
+      if (!(-pd <= S.radius[i])) continue;
+      S.wcf_active[c] = 1;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) S.wcf_shear[3 * c + k] = 0.0;
+    }
+    double R[9], Ri[9];
+    rot_to_x(nrm, R);
+    const double delta_n = pd + S.radius[i];
+    const double distance = S.radius[i] - delta_n;
+    if (delta_n > 0.05 * S.radius[i]) atomicAdd(S.status + 3, 1);
+    double ri[3], vci[3], mv[3], vc[3], F[3], sh[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ri[k] = distance * nrm[k];
+      S.wcf_pos[3 * c + k] = x[k] + ri[k];
+      sh[k] = S.wcf_shear[3 * c + k];
+    }
+    cross3(S.omega + 3 * i, ri, vci);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mv[k] = -(vci[k] + S.vel[3 * i + k]);
+    matvec(R, mv, vc);
+    contact_law(P, S, i, w, true, delta_n, vc, sh, F);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      S.wcf_shear[3 * c + k] = sh[k];
+      S.wcf_force[3 * c + k] = F[k];
+    }
+    inv3(R, Ri);
+    const double mF[3] = {-F[0], -F[1], -F[2]};
+    double Fi[3], Mi[3];
+    matvec(Ri, mF, Fi);
+    cross3(ri, Fi, Mi);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      S.force[3 * i + k] += Fi[k];
+      S.moment[3 * i + k] += Mi[k];
+    }
+  }
+}
+
+// quat2RotMatrix (demsolver/utils.py:34-61)
+__device__ __forceinline__ void quat_rot(const double *q, double *R) {
+  const double w2 = q[0] * q[0], i2 = q[1] * q[1], j2 = q[2] * q[2], k2 = q[3] * q[3];
+  const double ij = 2.0 * q[1] * q[2], ik = 2.0 * q[1] * q[3], jk = 2.0 * q[2] * q[3];
+  const double iw = 2.0 * q[1] * q[0], jw = 2.0 * q[2] * q[0], kw = 2.0 * q[3] * q[0];
+  R[0] = w2 + i2 - j2 - k2;
+  R[1] = ij - kw;
+  R[2] = jw + ik;
+  R[3] = ij + kw;
+  R[4] = w2 - i2 + j2 - k2;
+  R[5] = jk - iw;
+  R[6] = ik - jw;
+  R[7] = jk + iw;
+  R[8] = w2 - i2 - j2 + k2;
+}
+
+__device__ __forceinline__ void wrap_position(const lbdem_dem_params &P, double *x) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (P.periodic[a]) {
+      const double L = P.dmax[a] - P.dmin[a];
+      if (x[a] > P.dmax[a]) x[a] -= L;
+      else if (x[a] < P.dmin[a]) x[a] += L;
+    }
+}
+
+__device__ void rotate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
+  const double dt = P.dt;
+  double R[9], Ri[9], Ii[9], Ml[3], wl[3], Iw[3], cr[3], t[3], wdl[3], alpha[3];
+  quat_rot(S.quat + 4 * i, R);
+  matvec(R, S.moment + 3 * i, Ml);
+  matvec(R, S.omega + 3 * i, wl);
+  matvec(S.inertia + 9 * i, wl, Iw);
+  cross3(wl, Iw, cr);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) t[k] = Ml[k] - cr[k];
+  inv3(S.inertia + 9 * i, Ii);
+  matvec(Ii, t, wdl);
+  inv3(R, Ri);
+  matvec(Ri, wdl, alpha);
+  double *w = S.omega + 3 * i, *q = S.quat + 4 * i;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w[k] += 0.5 * (S.omega_dot[3 * i + k] + alpha[k]) * dt;
+  const double dq0 = -0.5 * (q[1] * w[0] + q[2] * w[1] + q[3] * w[2]);
+  const double dq1 = +0.5 * (q[0] * w[0] - q[3] * w[1] + q[2] * w[2]);
+  const double dq2 = +0.5 * (q[3] * w[0] + q[0] * w[1] + q[1] * w[2]);
+  const double dq3 = +0.5 * (-q[2] * w[0] + q[1] * w[1] + q[0] * w[2]);
+  q[0] += dq0 * dt;
+  q[1] += dq1 * dt;
+  q[2] += dq2 * dt;
+  q[3] += dq3 * dt;
+  const double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] /= qn;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) S.omega_dot[3 * i + k] = alpha[k];
+}
+
+// walls, apply_body_force (demsolver.py:437-448) and update (:514-636) for particle i
+__device__ void walls_and_integrate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
+  resolve_walls(P, S, i);
+  const double dt = P.dt;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    S.force[3 * i + k] += S.mass[i] * P.g[k];
+    S.force[3 * i + k] += S.force_fluid[3 * i + k];
+    S.moment[3 * i + k] += S.moment_fluid[3 * i + k];
+  }
+  if (S.freeze && S.freeze[i]) return;
+  double *x = S.pos + 3 * i, *v = S.vel + 3 * i;
+  if (S.fixvel && S.fixvel[i]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] += v[k] * dt;
+    wrap_position(P, x);
+    rotate(P, S, i);
+    return;
+  }
+  double a[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) a[k] = S.force[3 * i + k] / S.mass[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] += 0.5 * (S.acc[3 * i + k] + a[k]) * dt;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) x[k] += v[k] * dt + 0.5 * a[k] * dt * dt;
+  wrap_position(P, x);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) S.acc[3 * i + k] = a[k];
+  rotate(P, S, i);
+}
+
+// late_clear_state / remove_inactive_contact (demsolver.py:371-386, 431-436)
+__device__ void compact_slots(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
+  const int K = P.K;
+  const int m = min(S.cfn[i], K);
+  int act = 0;
+  for (int s = 0; s < m; ++s) act += S.cf_active[i * K + s] ? 1 : 0;
+  int o = 0;
+  for (int s = 0; s < m && o < act; ++s) {
+    const int src = i * K + s;
+    if (!S.cf_active[src]) continue;
+    const int dst = i * K + o;
+    if (dst != src) {
+      S.cf_j[dst] = S.cf_j[src];
+      S.cf_active[dst] = S.cf_active[src];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        S.cf_pos[3 * dst + k] = S.cf_pos[3 * src + k];
+        S.cf_force[3 * dst + k] = S.cf_force[3 * src + k];
+        S.cf_shear[3 * dst + k] = S.cf_shear[3 * src + k];
+      }
+    }
+    ++o;
+  }
+  for (int s = act; s < m; ++s) S.cf_active[i * K + s] = 0;
+  S.cfn[i] = act;
+}
+
+// --------------------------------------------------------------------------------------
+// general-path kernels
+// --------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dem_resolve_existing(const lbdem_dem_params P, const lbdem_dem_state S,
+                                                              const int32_t *pairs, const unsigned long long *npairs,
+                                                              int64_t cap, int32_t *need_new) {
+  const int64_t np = min((int64_t)*npairs, cap);
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < np; p += (int64_t)gridDim.x * 256)
+    need_new[p] = resolve_existing(P, S, pairs[2 * p], pairs[2 * p + 1]);
+}
+__global__ __launch_bounds__(256) void k_dem_resolve_new(const lbdem_dem_params P, const lbdem_dem_state S,
+                                                         const int32_t *pairs, const unsigned long long *npairs,
+                                                         int64_t cap, const int32_t *need_new) {
+  const int64_t np = min((int64_t)*npairs, cap);
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < np; p += (int64_t)gridDim.x * 256)
+    if (need_new[p]) resolve_new(P, S, pairs[2 * p], pairs[2 * p + 1]);
+}
+__global__ __launch_bounds__(256) void k_dem_integrate(const lbdem_dem_params P, const lbdem_dem_state S,
+                                                       const unsigned long long *npairs, int64_t cap) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0 && *npairs > (unsigned long long)cap) atomicMax(S.status + 4, (int)min(*npairs, 0x7fffffffull));
+  if (i >= P.n) return;
+  walls_and_integrate(P, S, i);
+}
+__global__ __launch_bounds__(256) void k_dem_compact(const lbdem_dem_params P, const lbdem_dem_state S,
+                                                     unsigned long long *npairs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i == 0 && npairs) *npairs = 0ull;
+  if (i >= P.n) return;
+  compact_slots(P, S, i);
+}
+
+// --------------------------------------------------------------------------------------
+// small-system path: one workgroup, `nsub` substeps per launch
+// --------------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_dem_small(const lbdem_dem_params P, const lbdem_dem_state S, int nsub,
+                                                    int32_t *probe, int32_t *cell_hash, int32_t *newlist,
+                                                    int newcap) {
+  const int n = P.n;
+  const int np = P.probe27 ? 27 : 8;
+  __shared__ int nnew;
+  for (int sub = 0; sub < nsub; ++sub) {
+    // clear_state + probe hashes of every particle
+    for (int i = threadIdx.x; i < n; i += 1024) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) S.force[3 * i + a] = S.moment[3 * i + a] = 0.0;
+      int32_t h[27];
+      probe_hashes(P, S.pos + 3 * i, h);
+      for (int k = 0; k < np; ++k) probe[i * 27 + k] = h[k];
+      int32_t c[3];
+      cell_of(P, S.pos + 3 * i, c);
+      cell_hash[i] = hash3(c[0], c[1], c[2], P.T);
+    }
+    if (threadIdx.x == 0) nnew = 0;
+    __threadfence();
+    __syncthreads();
+    // candidate pairs: j's bucket is one of i's probed buckets; existing contacts first
+    const long long npair = (long long)n * (n - 1) / 2;
+    for (long long t = threadIdx.x; t < npair; t += 1024) {
+      // t -> (i, j), i < j, row-major over the upper triangle
+      int i = (int)((2.0 * n - 1.0 - sqrt((2.0 * n - 1.0) * (2.0 * n - 1.0) - 8.0 * (double)t)) * 0.5);
+      long long row0 = (long long)i * (2LL * n - i - 1) / 2;
+      while (row0 > t) {
+        --i;
+        row0 = (long long)i * (2LL * n - i - 1) / 2;
+      }
+      while ((long long)(i + 1) * (2LL * n - i - 2) / 2 <= t) {
+        ++i;
+        row0 = (long long)i * (2LL * n - i - 1) / 2;
+      }
+      const int j = i + 1 + (int)(t - row0);
+      const int32_t hj = cell_hash[j];
+      bool cand = false;
+      for (int k = 0; k < np; ++k) cand |= (probe[i * 27 + k] == hj);
+      if (!cand) continue;
+      if (resolve_existing(P, S, i, j)) {
+        const int slot = atomicAdd(&nnew, 1);
+        if (slot < newcap) {
+          newlist[2 * slot] = i;
+          newlist[2 * slot + 1] = j;
+        }
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    const int nn = min(nnew, newcap);
+    if (threadIdx.x == 0 && nnew > newcap) atomicAdd(S.status + 0, nnew - newcap);
+    for (int t = threadIdx.x; t < nn; t += 1024) resolve_new(P, S, newlist[2 * t], newlist[2 * t + 1]);
+    __threadfence();
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) walls_and_integrate(P, S, i);
+    __threadfence();
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) compact_slots(P, S, i);
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+static int check_params(const lbdem_dem_params *P, const char *who) {
+  LBDEM_REQUIRE(P, "%s: null params", who);
+  LBDEM_REQUIRE(P->n >= 0, "%s: negative particle count", who);
+  LBDEM_REQUIRE(P->T > 0 && (P->T & (P->T - 1)) == 0, "%s: hash table size %d is not a power of two", who, P->T);
+  LBDEM_REQUIRE(P->cell > 0.0, "%s: cell size must be positive", who);
+  LBDEM_REQUIRE(P->K > 0, "%s: max_coordinate_number must be positive", who);
+  LBDEM_REQUIRE(P->model == 0 || P->model == 1, "%s: model must be 0 (Hertz) or 1 (linear)", who);
+  return LBDEM_OK;
+}
+
+static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lbdem_dem_work *w, double *force,
+                          double *moment, hipStream_t s, const char *who) {
+  const int n = P->n;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  const int ntiles = (P->T + 1023) / 1024;
+  hipLaunchKernelGGL(k_bp_count, dim3(nb), dim3(256), 0, s, *P, pos, w->hash, w->count, force, moment);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, w->count, w->offset, w->tile_sum, P->T);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, ntiles);
+  hipLaunchKernelGGL(k_scan_add, dim3((P->T + 255) / 256), dim3(256), 0, s, w->offset, w->tile_sum, P->T);
+  hipLaunchKernelGGL(k_bp_scatter, dim3(nb), dim3(256), 0, s, *P, w->hash, w->offset, w->current, w->pid);
+  hipLaunchKernelGGL(k_bp_probe, dim3(nb), dim3(256), 0, s, *P, pos, w->count, w->offset, w->pid, w->pairs,
+                     w->pair_cap, w->npairs);
+  hipLaunchKernelGGL(k_bp_reset, dim3(nb), dim3(256), 0, s, *P, w->hash, w->count, w->current);
+  LBDEM_CHECK_LAUNCH(who);
+  return LBDEM_OK;
+}
+
+}  // namespace lbdem
+
+using namespace lbdem;
+
+extern "C" {
+
+int lbdem_dem_broadphase(const lbdem_dem_params *P, const double *pos, const lbdem_dem_work *w, void *stream) {
+  int rc = check_params(P, "lbdem_dem_broadphase");
+  if (rc) return rc;
+  LBDEM_REQUIRE(w && w->count && w->offset && w->current && w->hash && w->pid && w->tile_sum && w->npairs &&
+                    (w->pairs || w->pair_cap == 0),
+                "lbdem_dem_broadphase: incomplete workspace");
+  LBDEM_REQUIRE(P->n == 0 || pos, "lbdem_dem_broadphase: pos is required");
+  if (P->n == 0) return LBDEM_OK;
+  return run_broadphase(P, pos, w, nullptr, nullptr, as_stream(stream), "lbdem_dem_broadphase");
+}
+
+int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbdem_dem_work *w, int32_t nsub,
+                  int32_t path, void *stream) {
+  int rc = check_params(P, "lbdem_dem_run");
+  if (rc) return rc;
+  LBDEM_REQUIRE(S && w, "lbdem_dem_run: null state or workspace");
+  LBDEM_REQUIRE(nsub >= 0, "lbdem_dem_run: negative substep count");
+  LBDEM_REQUIRE(S->status, "lbdem_dem_run: status buffer is required");
+  const int n = P->n;
+  if (n == 0 || nsub == 0) return LBDEM_OK;
+  hipStream_t s = as_stream(stream);
+  const bool small = (path == 1) || (path == 0 && n <= kSmallN);
+  if (small) {
+    LBDEM_REQUIRE(n <= kSmallN, "lbdem_dem_run: the single-workgroup path takes at most %d particles", kSmallN);
+    LBDEM_REQUIRE(w->probe && w->hash && w->pairs && w->pair_cap > 0,
+                  "lbdem_dem_run: small path needs probe, hash and pairs");
+    const int newcap = (int)min(w->pair_cap, (int64_t)0x7fffffff);
+    hipLaunchKernelGGL(k_dem_small, dim3(1), dim3(1024), 0, s, *P, *S, nsub, w->probe, w->hash, w->pairs, newcap);
+    LBDEM_CHECK_LAUNCH("lbdem_dem_run(small)");
+    return LBDEM_OK;
+  }
+  LBDEM_REQUIRE(w->need_new && w->pairs && w->pair_cap > 0, "lbdem_dem_run: need_new and pairs are required");
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  const unsigned pb = 2048;  // grid-stride over the candidate list
+  for (int sub = 0; sub < nsub; ++sub) {
+    rc = run_broadphase(P, S->pos, w, S->force, S->moment, s, "lbdem_dem_run");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dem_resolve_existing, dim3(pb), dim3(256), 0, s, *P, *S, w->pairs, w->npairs, w->pair_cap,
+                       w->need_new);
+    hipLaunchKernelGGL(k_dem_resolve_new, dim3(pb), dim3(256), 0, s, *P, *S, w->pairs, w->npairs, w->pair_cap,
+                       w->need_new);
+    hipLaunchKernelGGL(k_dem_integrate, dim3(nb), dim3(256), 0, s, *P, *S, w->npairs, w->pair_cap);
+    hipLaunchKernelGGL(k_dem_compact, dim3(nb), dim3(256), 0, s, *P, *S, w->npairs);
+    LBDEM_CHECK_LAUNCH("lbdem_dem_run");
+  }
+  return LBDEM_OK;
+}
+
+}  // extern "C"

@@ -167,11 +167,12 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
     T omega_old = T(0);
     if constexpr (COLL == LBDEM_COLL_SMAG_LAGGED) {
       const size_t N = (size_t)p.ncell;
-      er = (T)p.lag[c];
-      eux = (T)p.lag[N + c];
-      euy = (T)p.lag[2 * N + c];
-      euz = (T)p.lag[3 * N + c];
-      omega_old = (T)p.lag[4 * N + c];
+      const T *lag = reinterpret_cast<const T *>(p.lag);
+      er = lag[c];
+      eux = lag[N + c];
+      euy = lag[2 * N + c];
+      euz = lag[3 * N + c];
+      omega_old = lag[4 * N + c];
     } else {
       er = rho;
       eux = ux;
@@ -192,11 +193,12 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
       const T rate = omega_old * Pi * T(1.5) * inv;
       omega = T(1) / (T(p.tau0) + T(3) * T(p.cs2_smag) * rate);
       const size_t N = (size_t)p.ncell;
-      p.lag[c] = (float)rho;
-      p.lag[N + c] = (float)ux;
-      p.lag[2 * N + c] = (float)uy;
-      p.lag[3 * N + c] = (float)uz;
-      p.lag[4 * N + c] = (float)omega;
+      T *lag = reinterpret_cast<T *>(p.lag);
+      lag[c] = rho;
+      lag[N + c] = ux;
+      lag[2 * N + c] = uy;
+      lag[3 * N + c] = uz;
+      lag[4 * N + c] = omega;
     } else {
       // tau = tau0 + 9/2 Cs^2 Pi / (rho tau)  =>  tau = (tau0 + sqrt(tau0^2 + 18 Cs^2 Pi/rho)) / 2
       const T tau0 = T(p.tau0);
@@ -400,11 +402,12 @@ __global__ __launch_bounds__(256) void k_init(const KParams p) {
     Store<S>::st(F + q * N + c, T(kW[q]) * (dr + rho * fvf * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq)));
   });
   if (p.lag) {
-    p.lag[c] = (float)rho;
-    p.lag[N + c] = (float)ux;
-    p.lag[2 * N + c] = (float)uy;
-    p.lag[3 * N + c] = (float)uz;
-    p.lag[4 * N + c] = (float)p.omega0;
+    T *lag = reinterpret_cast<T *>(p.lag);
+    lag[c] = rho;
+    lag[N + c] = ux;
+    lag[2 * N + c] = uy;
+    lag[3 * N + c] = uz;
+    lag[4 * N + c] = (T)p.omega0;
   }
 }
 
@@ -517,6 +520,239 @@ __global__ __launch_bounds__(256) void k_reduce_partial(const KParams p) {
   if (threadIdx.x < 5) {
     const int m = threadIdx.x;
     p.work[(size_t)blockIdx.x * 5 + m] = ((sh[0][m] + sh[1][m]) + (sh[2][m] + sh[3][m]));
+  }
+}
+
+}  // namespace lbdem
+
+// ---------------------------------------------------------------------------------------
+// Pencil-tiled variant: one workgroup per (i, j) row of nz cells, V consecutive cells per
+// thread.  Every global load and store of a step covers whole aligned row segments: the
+// x/y neighbours of a row are other rows (same k range), and the +-1 shift along k of the
+// odd step is applied by staging the ten z-moving populations through LDS, with periodic
+// wrap inside the row.  Used when nz <= 1024 V and nz % V == 0.
+// ---------------------------------------------------------------------------------------
+namespace lbdem {
+
+// index of a z-moving direction among {5, 6, 11, ..., 18}
+template <int q>
+constexpr int zslot() {
+  return q == 5 ? 0 : q == 6 ? 1 : q - 9;
+}
+constexpr int kNZ = 10;
+
+// V consecutive elements (V = 1, or V = 2 for 16-bit storage) as one access
+template <typename S, int V>
+struct Vec {
+  using T = typename Store<S>::T;
+  static __device__ __forceinline__ void ld(const S *p, T (&v)[V]) {
+    if constexpr (V == 1) {
+      v[0] = Store<S>::ld(p);
+    } else {
+      static_assert(sizeof(S) == 2 && V == 2, "pairs of 16-bit values only");
+      const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+      S a, b;
+      __builtin_memcpy(&a, &w, 2);
+      const uint32_t hi = w >> 16;
+      __builtin_memcpy(&b, &hi, 2);
+      v[0] = Store<S>::ld(&a);
+      v[1] = Store<S>::ld(&b);
+    }
+  }
+  static __device__ __forceinline__ void st(S *p, const T (&v)[V]) {
+    if constexpr (V == 1) {
+      Store<S>::st(p, v[0]);
+    } else {
+      S a = (S)v[0], b = (S)v[1];
+      uint16_t ua, ub;
+      __builtin_memcpy(&ua, &a, 2);
+      __builtin_memcpy(&ub, &b, 2);
+      __builtin_nontemporal_store((uint32_t)ua | ((uint32_t)ub << 16), reinterpret_cast<uint32_t *>(p));
+    }
+  }
+};
+
+template <typename S, int COLL, int COUP, bool ODD, int V>
+__global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
+  using T = typename Store<S>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *buf = reinterpret_cast<T *>(smem);  // [kNZ][ncz]
+  const int nz = p.nz;
+  const int ncz = blockDim.x * V;         // row cells covered by the block (>= nz)
+  unsigned char *okb = smem + sizeof(T) * kNZ * ncz;  // [kNZ][ncz] push validity
+  const int row = blockIdx.x;
+  const int i = row / p.ny, j = row - (row / p.ny) * p.ny;
+  const int k0 = threadIdx.x * V;
+  const bool lane_in = k0 < nz;
+  S *__restrict__ F = reinterpret_cast<S *>(p.f);
+  const size_t N = (size_t)p.ncell;
+  const int base = row * nz;
+
+  uint8_t fl[V];
+  bool act[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    fl[v] = lane_in ? p.flags[base + k0 + v] : (uint8_t)0;
+    act[v] = fl[v] & LBDEM_FLAG_ACTIVE;
+  }
+  T f[V][Q];
+
+  // ---------------------------------------------------------------- gather
+  if constexpr (!ODD) {
+    if (lane_in) {
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        T t[V];
+        Vec<S, V>::ld(F + q * N + base + k0, t);
+#pragma unroll
+        for (int v = 0; v < V; ++v) f[v][q] = t[v];
+      });
+    }
+  } else {
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      const int srow = wrapi(i - cx, p.nx) * p.ny + wrapi(j - cy, p.ny);
+      if (lane_in) {
+        T t[V];
+        Vec<S, V>::ld(F + kOPP[q] * N + (size_t)srow * nz + k0, t);
+        if constexpr (cz == 0) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) f[v][q] = t[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) buf[zslot<q>() * ncz + k0 + v] = t[v];
+        }
+      }
+    });
+    __syncthreads();
+    if (lane_in) {
+      static_for<1, Q>([&](auto qc) {
+        constexpr int q = qc;
+        constexpr int cz = kCZ[q];
+        if constexpr (cz != 0) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            int ks = k0 + v - cz;
+            ks = ks < 0 ? ks + nz : (ks >= nz ? ks - nz : ks);
+            f[v][q] = buf[zslot<q>() * ncz + ks];
+          }
+        }
+      });
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (fl[v] & LBDEM_FLAG_NEAR) {  // populations bounced back at a wall last step
+          const int k = k0 + v;
+          static_for<1, Q>([&](auto qc) {
+            constexpr int q = qc;
+            const int n = nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
+            if (p.flags[n] & kMaskSolid) f[v][q] = Store<S>::ld(F + q * N + base + k);
+          });
+        }
+      }
+    }
+  }
+
+  // ---------------------------------------------------------------- collide
+  T rho[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    rho[v] = T(0);
+    if (act[v]) rho[v] = collide<COLL, COUP, T>(p, base + k0 + v, i, j, k0 + v, f[v]);
+  }
+
+  // ---------------------------------------------------------------- scatter
+  if constexpr (!ODD) {
+    if (lane_in) {
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        T t[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          t[v] = f[v][q];
+          if (q != 0 && (fl[v] & LBDEM_FLAG_NEAR)) {
+            const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
+            if (p.flags[m] & LBDEM_FLAG_MOVING) t[v] -= ladd_term<q, T>(p, m, rho[v]);
+          }
+        }
+        // inactive cells' slots are never read, so they may receive garbage here
+        Vec<S, V>::st(F + kOPP[q] * N + base + k0, t);
+      });
+    }
+  } else {
+    __syncthreads();  // the gather's LDS reads are done
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
+      if (lane_in) {
+        bool ok[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          ok[v] = act[v];
+          if (q != 0 && act[v] && (fl[v] & LBDEM_FLAG_NEAR)) {
+            const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
+            const uint8_t fm = p.flags[m];
+            if (fm & kMaskSolid) {  // bounce back into the own opposite slot
+              T val = f[v][q];
+              if (fm & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
+              Store<S>::st(F + kOPP[q] * N + base + k0 + v, val);
+              ok[v] = false;
+            }
+          }
+        }
+        if constexpr (cz == 0) {
+          S *dst = F + q * N + (size_t)trow * nz + k0;
+          bool all = true;
+#pragma unroll
+          for (int v = 0; v < V; ++v) all = all && ok[v];
+          if (all) {
+            T t[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) t[v] = f[v][q];
+            Vec<S, V>::st(dst, t);
+          } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+              if (ok[v]) Store<S>::st(dst + v, f[v][q]);
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            buf[zslot<q>() * ncz + k0 + v] = f[v][q];
+            okb[zslot<q>() * ncz + k0 + v] = ok[v] ? 1 : 0;
+          }
+        }
+      }
+    });
+    __syncthreads();
+    if (lane_in) {
+      static_for<1, Q>([&](auto qc) {
+        constexpr int q = qc;
+        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+        if constexpr (cz != 0) {
+          const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
+          S *dst = F + q * N + (size_t)trow * nz + k0;
+          T t[V];
+          bool ok[V], all = true;
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            int ks = k0 + v - cz;  // the cell pushing into (row trow, k0 + v)
+            ks = ks < 0 ? ks + nz : (ks >= nz ? ks - nz : ks);
+            t[v] = buf[zslot<q>() * ncz + ks];
+            ok[v] = okb[zslot<q>() * ncz + ks] != 0;
+            all = all && ok[v];
+          }
+          if (all) {
+            Vec<S, V>::st(dst, t);
+          } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+              if (ok[v]) Store<S>::st(dst + v, t[v]);
+          }
+        }
+      });
+    }
   }
 }
 

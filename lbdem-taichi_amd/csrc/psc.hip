@@ -1,0 +1,294 @@
+// Partially saturated cells (resolved LBM-DEM coupling): grain -> lattice mapping and the
+// per-grain FP64 hydrodynamic force / torque reduction.  The weighted collision itself is
+// folded into the collide-stream kernel (coupling LBDEM_COUPLE_PSC).
+//
+// Mapping is grain-centric: one workgroup per grain walks the grain's bounding box and
+// publishes, per covered cell, a 32-bit key with atomicMax:
+//     bit 30      full coverage (dist <= r - sqrt(3)/2)
+//     bits 29..23 sub-cell count (1..125) of a partially covered cell
+//     bits 22..0  0x7FFFFF - grain index
+// so the maximum key is exactly the reference's serial rule (cell loop over grains in
+// index order, psclattice3d.py:95-138): the lowest-index full cover wins; otherwise the
+// largest count, ties to the lowest index.
+#include "common.hpp"
+
+namespace lbdem {
+
+constexpr uint32_t kFullBit = 1u << 30;
+constexpr int kMaxGrain = 0x7FFFFF;
+
+struct PscArgs {
+  int nx, ny, nz, ngrain;
+  const uint8_t *flags;
+  const double *pos, *vel, *omg, *rad;
+  double dx, dt, rho, dmin0, dmin1, dmin2;
+  double tau0;
+  uint32_t *key;
+  float *snap;   // 12 per grain
+  int *bbox;     // 6 per grain: i0, i1, j0, j1, k0, k1 (half-open)
+  const float *cell_force;
+  double *force_fluid, *moment_fluid;
+};
+
+// bounding box of lattice2grains (psclattice3d.py:326-337): [int(xc - r), int(xc + r + 2))
+// with r = R/dx, clamped; it contains every cell the mapping can touch.
+__device__ __forceinline__ void grain_box(const PscArgs &a, int g, double &xc, double &yc, double &zc, int b[6]) {
+  xc = (a.pos[3 * g] - a.dmin0 + 0.5 * a.dx) / a.dx;
+  yc = (a.pos[3 * g + 1] - a.dmin1 + 0.5 * a.dx) / a.dx;
+  zc = (a.pos[3 * g + 2] - a.dmin2 + 0.5 * a.dx) / a.dx;
+  const double r = a.rad[g] / a.dx;
+  b[0] = max((int)(xc - r), 0);
+  b[1] = min((int)(xc + r + 2), a.nx);
+  b[2] = max((int)(yc - r), 0);
+  b[3] = min((int)(yc + r + 2), a.ny);
+  b[4] = max((int)(zc - r), 0);
+  b[5] = min((int)(zc + r + 2), a.nz);
+}
+
+// zero the keys written by the previous mapping (its boxes are in bbox)
+__global__ __launch_bounds__(256) void k_psc_clear(const PscArgs a) {
+  const int g = blockIdx.x;
+  const int *b = a.bbox + 6 * g;
+  const int ni = b[1] - b[0], nj = b[3] - b[2], nk = b[5] - b[4];
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  const int tot = ni * nj * nk;
+  for (int t = threadIdx.x; t < tot; t += 256) {
+    const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
+    a.key[(i * a.ny + j) * a.nz + k] = 0u;
+  }
+}
+
+// grains2lattice (psclattice3d.py:84-182), grain-centric
+__global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
+  const int g = blockIdx.x;
+  double xc, yc, zc;
+  int b[6];
+  grain_box(a, g, xc, yc, zc, b);
+  const double r = (a.rad[g] - 0.5 * a.dx) / a.dx;
+  const double half_diag = 0.5 * sqrt(3.0);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) a.bbox[6 * g + m] = b[m];
+    // snapshot for the collision: centre (lattice units), v*dt/dx, omega*dt
+    float *s = a.snap + 12 * g;
+    s[0] = (float)xc;
+    s[1] = (float)yc;
+    s[2] = (float)zc;
+    const double vs = a.dt / a.dx;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      s[3 + m] = (float)(a.vel[3 * g + m] * vs);
+      s[6 + m] = (float)(a.omg[3 * g + m] * a.dt);
+      s[9 + m] = 0.f;
+    }
+  }
+  const int ni = b[1] - b[0], nj = b[3] - b[2], nk = b[5] - b[4];
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  const int tot = ni * nj * nk;
+  const uint32_t gid = (uint32_t)(kMaxGrain - g);
+  for (int t = threadIdx.x; t < tot; t += 256) {
+    const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
+    const int c = (i * a.ny + j) * a.nz + k;
+    if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) continue;
+    const double dist = sqrt((xc - i) * (xc - i) + (yc - j) * (yc - j) + (zc - k) * (zc - k));
+    if (dist >= r + half_diag) continue;
+    uint32_t key;
+    if (dist <= r - half_diag) {
+      key = kFullBit | gid;
+    } else {
+      int cnt = 0;
+      for (int ii = 0; ii < 5; ++ii) {
+        const double px = (double)i - 0.4 + 0.2 * ii;
+        const double dxx = (xc - px) * (xc - px);
+        for (int jj = 0; jj < 5; ++jj) {
+          const double py = (double)j - 0.4 + 0.2 * jj;
+          const double dyy = (yc - py) * (yc - py);
+          for (int kk = 0; kk < 5; ++kk) {
+            const double pz = (double)k - 0.4 + 0.2 * kk;
+            const double d2 = sqrt(dxx + dyy + (zc - pz) * (zc - pz));
+            cnt += (d2 < r) ? 1 : 0;
+          }
+        }
+      }
+      if (cnt == 0) continue;
+      key = ((uint32_t)cnt << 23) | gid;
+    }
+    atomicMax(a.key + c, key);
+  }
+}
+
+__device__ __forceinline__ double block_sum256(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// lattice2grains (psclattice3d.py:316-358): FP64, fixed reduction order (deterministic)
+__global__ __launch_bounds__(256) void k_psc_forces(const PscArgs a) {
+  const int g = blockIdx.x;
+  double xc, yc, zc;
+  int b[6];
+  grain_box(a, g, xc, yc, zc, b);
+  const double scale = a.rho * a.dx * a.dx * a.dx * a.dx / (a.dt * a.dt);
+  double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+  const int ni = b[1] - b[0], nj = b[3] - b[2], nk = b[5] - b[4];
+  const uint32_t gid = (uint32_t)(kMaxGrain - g);
+  if (ni > 0 && nj > 0 && nk > 0) {
+    const int tot = ni * nj * nk;
+    for (int t = threadIdx.x; t < tot; t += 256) {
+      const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
+      const int c = (i * a.ny + j) * a.nz + k;
+      const uint32_t key = a.key[c];
+      if (key == 0u || (key & 0x7FFFFFu) != gid) continue;
+      const double f0 = (double)a.cell_force[3 * c] * scale, f1 = (double)a.cell_force[3 * c + 1] * scale,
+                   f2 = (double)a.cell_force[3 * c + 2] * scale;
+      const double d0 = (xc - i) * a.dx, d1 = (yc - j) * a.dx, d2 = (zc - k) * a.dx;
+      F[0] += f0;
+      F[1] += f1;
+      F[2] += f2;
+      T[0] -= d1 * f2 - d2 * f1;
+      T[1] -= d2 * f0 - d0 * f2;
+      T[2] -= d0 * f1 - d1 * f0;
+    }
+  }
+  __shared__ double sh[4];
+  double out[6];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) out[m] = block_sum256(F[m], sh);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) out[3 + m] = block_sum256(T[m], sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      a.force_fluid[3 * g + m] = out[m];
+      a.moment_fluid[3 * g + m] = out[3 + m];
+    }
+  }
+}
+
+// id / volfrac / weight / velsolid fields decoded from the keys (for inspection)
+__global__ __launch_bounds__(256) void k_psc_decode(const PscArgs a, int *id, float *volfrac, float *weight,
+                                                    float *velsolid, int ncell) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncell) return;
+  const uint32_t key = a.key[c];
+  if (key == 0u) {
+    id[c] = -1;
+    volfrac[c] = 0.f;
+    weight[c] = 0.f;
+    velsolid[3 * c] = velsolid[3 * c + 1] = velsolid[3 * c + 2] = 0.f;
+    return;
+  }
+  const int g = kMaxGrain - (int)(key & 0x7FFFFFu);
+  const int nyz = a.ny * a.nz;
+  const int i = c / nyz, j = (c - i * nyz) / a.nz, k = c - i * nyz - j * a.nz;
+  id[c] = g;
+  float eps, B;
+  if (key & kFullBit) {
+    eps = 1.f;
+    B = 1.f;
+  } else {
+    eps = (float)((key >> 23) & 0x7Fu) * (1.0f / 125.0f);
+    const float th = (float)a.tau0 - 0.5f;
+    B = eps * th / ((1.f - eps) + th);
+  }
+  volfrac[c] = eps;
+  weight[c] = B;
+  const float *s = a.snap + 12 * g;
+  const float rx = i - s[0], ry = j - s[1], rz = k - s[2];
+  velsolid[3 * c] = s[3] + (s[7] * rz - s[8] * ry);
+  velsolid[3 * c + 1] = s[4] + (s[8] * rx - s[6] * rz);
+  velsolid[3 * c + 2] = s[5] + (s[6] * ry - s[7] * rx);
+}
+
+static int fill_psc(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u, PscArgs &a,
+                    const char *who) {
+  LBDEM_REQUIRE(lat && g && u, "%s: null argument", who);
+  LBDEM_REQUIRE(g->n >= 0 && g->n <= kMaxGrain + 1, "%s: %d grains (at most %d)", who, g->n, kMaxGrain + 1);
+  LBDEM_REQUIRE(u->dx > 0 && u->dt > 0, "%s: dx and dt must be positive", who);
+  a = PscArgs{};
+  a.nx = lat->nx;
+  a.ny = lat->ny;
+  a.nz = lat->nz;
+  a.ngrain = g->n;
+  a.pos = g->position;
+  a.vel = g->velocity;
+  a.omg = g->omega;
+  a.rad = g->radius;
+  a.force_fluid = g->force_fluid;
+  a.moment_fluid = g->moment_fluid;
+  a.dx = u->dx;
+  a.dt = u->dt;
+  a.rho = u->rho;
+  a.dmin0 = u->dmin[0];
+  a.dmin1 = u->dmin[1];
+  a.dmin2 = u->dmin[2];
+  a.tau0 = 1.0 / lat->omega0;
+  return LBDEM_OK;
+}
+
+}  // namespace lbdem
+
+using namespace lbdem;
+
+extern "C" {
+
+int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g, const lbdem_units *u,
+                  uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous, void *stream) {
+  PscArgs a;
+  int rc = fill_psc(lat, g, u, a, "lbdem_psc_map");
+  if (rc) return rc;
+  LBDEM_REQUIRE(flags && key && snap && bbox, "lbdem_psc_map: flags, key, snap and bbox are required");
+  LBDEM_REQUIRE(g->n == 0 || (g->position && g->velocity && g->omega && g->radius),
+                "lbdem_psc_map: grain position, velocity, omega and radius are required");
+  a.flags = flags;
+  a.key = key;
+  a.snap = snap;
+  a.bbox = bbox;
+  if (g->n == 0) return LBDEM_OK;
+  if (clear_previous) {
+    hipLaunchKernelGGL(k_psc_clear, dim3(g->n), dim3(256), 0, as_stream(stream), a);
+    LBDEM_CHECK_LAUNCH("lbdem_psc_map(clear)");
+  }
+  hipLaunchKernelGGL(k_psc_map, dim3(g->n), dim3(256), 0, as_stream(stream), a);
+  LBDEM_CHECK_LAUNCH("lbdem_psc_map");
+  return LBDEM_OK;
+}
+
+int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u, const uint32_t *key,
+                     const float *cell_force, void *stream) {
+  PscArgs a;
+  int rc = fill_psc(lat, g, u, a, "lbdem_psc_forces");
+  if (rc) return rc;
+  LBDEM_REQUIRE(key && cell_force, "lbdem_psc_forces: key and cell_force are required");
+  LBDEM_REQUIRE(g->n == 0 || (g->position && g->radius && g->force_fluid && g->moment_fluid),
+                "lbdem_psc_forces: grain position, radius, force_fluid and moment_fluid are required");
+  if (g->n == 0) return LBDEM_OK;
+  a.key = const_cast<uint32_t *>(key);
+  a.cell_force = cell_force;
+  hipLaunchKernelGGL(k_psc_forces, dim3(g->n), dim3(256), 0, as_stream(stream), a);
+  LBDEM_CHECK_LAUNCH("lbdem_psc_forces");
+  return LBDEM_OK;
+}
+
+int lbdem_psc_decode(const lbdem_lattice *lat, const uint32_t *key, const float *snap, int32_t *id,
+                     float *volfrac, float *weight, float *velsolid, void *stream) {
+  LBDEM_REQUIRE(lat && key && snap && id && volfrac && weight && velsolid, "lbdem_psc_decode: null argument");
+  PscArgs a{};
+  a.nx = lat->nx;
+  a.ny = lat->ny;
+  a.nz = lat->nz;
+  a.key = const_cast<uint32_t *>(key);
+  a.snap = const_cast<float *>(snap);
+  a.tau0 = 1.0 / lat->omega0;
+  const int n = lat->nx * lat->ny * lat->nz;
+  hipLaunchKernelGGL(k_psc_decode, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream), a, id, volfrac,
+                     weight, velsolid, n);
+  LBDEM_CHECK_LAUNCH("lbdem_psc_decode");
+  return LBDEM_OK;
+}
+
+}  // extern "C"

@@ -24,7 +24,7 @@ class Lattice(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("storage", ctypes.c_int32), ("collision", ctypes.c_int32), ("coupling", ctypes.c_int32),
                 ("omega0", ctypes.c_double), ("cs_smag", ctypes.c_double), ("rho0", ctypes.c_double),
-                ("force", ctypes.c_double * 3)]
+                ("force", ctypes.c_double * 3), ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class StepArgs(ctypes.Structure):

@@ -77,6 +77,8 @@ class BasicLattice3D:
     # cells that collide() / stream() skip (lbm_solver3d.py:183)
     _skip_types = CellType.OBSTACLE | CellType.VEL_LADD | CellType.FREE_SLIP | CellType.VEL_INLET_LADD
     _coupling = _hip.COUPLE_NONE
+    # collide-stream variant: 0 automatic, 1 one cell per thread, 2 row-tiled
+    kernel = 0
 
     def __init__(self, Nx: int, Ny: int, Nz: int, omega: float, dx: float, dt: float, rho: float,
                  storage=None, smagorinsky=None):
@@ -103,8 +105,9 @@ class BasicLattice3D:
         self._flags = torch.zeros(self.N, dtype=torch.uint8, device=dev)
         self._lag = None
         if self._collision() == _hip.COLL_SMAG_LAGGED:
-            self._lag = torch.zeros(5 * self.N, dtype=torch.float32, device=dev)
-            self.omega = Field(shp, torch.float32, dev, tensor=self._lag[4 * self.N:].view(shp))
+            lag_dtype = torch.float64 if self.storage == "f64" else torch.float32
+            self._lag = torch.zeros(5 * self.N, dtype=lag_dtype, device=dev)
+            self.omega = Field(shp, lag_dtype, dev, tensor=self._lag[4 * self.N:].view(shp))
         else:
             self.omega = Field(shp, torch.float32, dev)
         self.omega.fill(self.omega0)
@@ -137,6 +140,7 @@ class BasicLattice3D:
         L.rho0 = 1.0
         fx, fy, fz = self._body_force()
         L.force[0], L.force[1], L.force[2] = float(fx), float(fy), float(fz)
+        L.kernel = int(self.kernel)
 
     def _coupling_args(self, a):
         """Subclasses fill the coupling members of the step arguments."""

@@ -10,7 +10,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblbdem_oracle.so")
-SRCS = ["lbm_oracle.c", "dem_oracle.c"]
+SRCS = ["lbm_oracle.c"]          # compiled for REAL = double and REAL = float
+SRCS_F64 = ["dem_oracle.c"]      # FP64 only
 CFLAGS = ["-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v2",
           "-std=c11", "-Wall", "-Wno-unused-function"]
 
@@ -30,6 +31,13 @@ def build(verbose=True):
             if r.returncode != 0:
                 raise RuntimeError(f"gcc failed for {src} ({real}):\n{r.stderr}")
             objs.append(obj)
+    for src in SRCS_F64:
+        obj = os.path.join(bdir, f"{src}.o")
+        r = subprocess.run(["gcc", *CFLAGS, "-D_GNU_SOURCE", "-c", os.path.join(HERE, src), "-o", obj],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"gcc failed for {src}:\n{r.stderr}")
+        objs.append(obj)
     r = subprocess.run(["gcc", "-shared", "-fopenmp", "-o", LIB + ".tmp", *objs, "-lm"], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle link failed:\n{r.stderr}")

@@ -239,3 +239,142 @@ def cavity(L, lid_u, lid_axis=2):
     CT[tuple(lid)] = VEL_LADD
     L.vel[tuple(lid)] = np.asarray(lid_u, dtype=L.real)
     return CT
+
+
+# ---------------------------------------------------------------------------------------
+# DEM restatement (dem_oracle.c)
+# ---------------------------------------------------------------------------------------
+_i32, _f64 = ctypes.c_int32, ctypes.c_double
+
+
+class DemParams(ctypes.Structure):
+    _fields_ = [("n", _i32), ("K", _i32), ("model", _i32), ("dt", _f64), ("g", _f64 * 3), ("dmin", _f64 * 3),
+                ("dmax", _f64 * 3), ("periodic", _i32 * 3), ("cell", _f64), ("T", _i32), ("probe27", _i32),
+                ("mat", (_f64 * 9) * 2), ("wall_n", (_f64 * 3) * 6), ("wall_d", _f64 * 6), ("wall_type", _i32 * 6)]
+
+
+_STATE_FIELDS = ["mass", "radius", "pos", "vel", "acc", "force", "quat", "omega", "omega_dot", "inertia", "moment",
+                 "force_fluid", "moment_fluid", "freeze", "fixvel", "cf_j", "cf_active", "cfn", "cf_pos", "cf_force",
+                 "cf_shear", "wcf_active", "wcf_pos", "wcf_force", "wcf_shear", "status"]
+
+
+class DemState(ctypes.Structure):
+    _fields_ = [(k, _dp) for k in _STATE_FIELDS]
+
+
+def morton3d32(x, y, z):
+    L = lib()
+    L.lbo_morton3d32.restype = ctypes.c_uint32
+    return L.lbo_morton3d32(_i32(x), _i32(y), _i32(z))
+
+
+def hash_table_size(dmin, dmax, cell):
+    L = lib()
+    L.lbo_hash_table_size.restype = ctypes.c_int64
+    a = np.ascontiguousarray(dmin, dtype=np.float64)
+    b = np.ascontiguousarray(dmax, dtype=np.float64)
+    return int(L.lbo_hash_table_size(P(a), P(b), _f64(cell)))
+
+
+def broadphase(pos, dmin, cell, T, probe27=False):
+    """Candidate pairs (i < j) of the reference's broad phase, sorted (m, 2) int32."""
+    L = lib()
+    L.lbo_broadphase.restype = ctypes.c_int64
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    n = len(pos)
+    cap = max(64 * n, 16)
+    while True:
+        pairs = np.zeros((cap, 2), dtype=np.int32)
+        m = L.lbo_broadphase(_i32(n), P(pos), P(dm), _f64(cell), _i32(T), _i32(1 if probe27 else 0), P(pairs),
+                             ctypes.c_int64(cap))
+        if m <= cap:
+            break
+        cap = m
+    pr = pairs[:m]
+    return pr[np.lexsort((pr[:, 1], pr[:, 0]))]
+
+
+def material_rows(cfg, density):
+    """mf[0] (particle) and mf[1] (wall) as DEMSolver.init_particle_fields sets them
+    (demsolver.py:222-290); row layout density, E, nu, mu, e, kn, kt, dpn, dps."""
+    cm = cfg.contact_model
+    pp, wp = cfg.particle_props, cfg.wall_props
+    m = np.zeros((2, 9))
+    m[0, :3] = density, pp.elastic_modulus, pp.poisson_ratio
+    m[1, :3] = wp.density, wp.elastic_modulus, wp.poisson_ratio
+    if cm.get_model_name() == "linear":
+        m[0, 5], m[0, 6], m[0, 7] = cm.stiffness_normal, cm.stiffness_tangential, cm.damping_normal
+        m[0, 8] = 0.0  # dp_sratio is never assigned (demsolver.py:250 writes dp_tratio)
+        m[0, 3], m[1, 3] = cm.pp_friction, cm.pw_friction
+        m[0, 4] = getattr(cm, "pp_restitution", 0.0)
+        m[1, 4] = getattr(cm, "pw_restitution", 0.0)
+    else:
+        m[0, 4], m[1, 4] = cm.pp_restitution, cm.pw_restitution
+        m[0, 3], m[1, 3] = cm.pp_friction, cm.pw_friction
+    return m
+
+
+class OracleDEM:
+    """DEMSolver state in reference semantics, stepped by the serial C restatement."""
+
+    def __init__(self, cfg, radius, mass, position, velocity=None, K=None, cell=None, probe27=False,
+                 dmin=None, dmax=None):
+        n = len(radius)
+        K = int(K if K is not None else cfg.particle_props.max_coordinate_number)
+        self.n, self.K = n, K
+        d = cfg.domain
+        dmin = np.array([d.xmin, d.ymin, d.zmin]) if dmin is None else np.asarray(dmin, dtype=np.float64)
+        dmax = np.array([d.xmax, d.ymax, d.zmax]) if dmax is None else np.asarray(dmax, dtype=np.float64)
+        radius = np.asarray(radius, dtype=np.float64)
+        mass = np.asarray(mass, dtype=np.float64)
+        cell = 4.0 * radius.max() if cell is None else cell
+        p = DemParams()
+        p.n, p.K = n, K
+        p.model = 1 if cfg.contact_model.get_model_name() == "linear" else 0
+        p.dt = cfg.dt
+        for a in range(3):
+            p.g[a] = float(cfg.gravity[a])
+            p.dmin[a] = float(dmin[a])
+            p.dmax[a] = [d.xmax, d.ymax, d.zmax][a]
+            p.periodic[a] = 1 if cfg.boundaryType[2 * a] == cfg.BOUNDARY_TYPE_PERIODIC else 0
+        p.cell = cell
+        p.T = hash_table_size(dmin, dmax, cell)
+        p.probe27 = 1 if probe27 else 0
+        density = 3 * mass[-1] / (4 * np.pi * radius[-1] ** 3)
+        mat = material_rows(cfg, density)
+        for k in range(2):
+            for m in range(9):
+                p.mat[k][m] = mat[k, m]
+        for j in range(6):
+            for a in range(3):
+                p.wall_n[j][a] = float(cfg.wall_normals[j][a])
+            p.wall_d[j] = float(cfg.wall_distances[j])
+            p.wall_type[j] = int(cfg.boundaryType[j])
+        self.params = p
+        z = lambda *s: np.zeros(s)  # noqa: E731
+        zi = lambda *s: np.zeros(s, dtype=np.int32)  # noqa: E731
+        self.a = dict(mass=mass.copy(), radius=radius.copy(), pos=np.array(position, dtype=np.float64).reshape(n, 3),
+                      vel=(z(n, 3) if velocity is None else np.array(velocity, dtype=np.float64).reshape(n, 3)),
+                      acc=z(n, 3), force=z(n, 3), quat=np.tile([1.0, 0, 0, 0], (n, 1)), omega=z(n, 3),
+                      omega_dot=z(n, 3), inertia=np.zeros((n, 3, 3)), moment=z(n, 3), force_fluid=z(n, 3),
+                      moment_fluid=z(n, 3), freeze=zi(n), fixvel=zi(n), cf_j=zi(n * K), cf_active=zi(n * K),
+                      cfn=zi(n), cf_pos=z(n * K, 3), cf_force=z(n * K, 3), cf_shear=z(n * K, 3),
+                      wcf_active=zi(6 * n), wcf_pos=z(6 * n, 3), wcf_force=z(6 * n, 3), wcf_shear=z(6 * n, 3),
+                      status=zi(8))
+        for i in range(3):
+            self.a["inertia"][:, i, i] = 2.0 / 5.0 * mass * radius * radius
+        self._pairs = np.zeros((max(256 * n, 16), 2), dtype=np.int32)
+
+    def step(self, n=1):
+        L = lib()
+        L.lbo_dem_step.restype = ctypes.c_int64
+        s = DemState()
+        for k in _STATE_FIELDS:
+            setattr(s, k, P(self.a[k]))
+        for _ in range(n):
+            L.lbo_dem_step(ctypes.byref(self.params), ctypes.byref(s), P(self._pairs),
+                           ctypes.c_int64(len(self._pairs)))
+
+    def __getitem__(self, k):
+        return self.a[k]

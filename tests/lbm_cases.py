@@ -7,6 +7,8 @@ from oracle import oracle as O
 def oracle_cavity(n, tau=0.6, lid=0.1, kind="basic", cs=0.17, real=np.float64):
     L = O.OracleLattice(n, n, n, 1.0 / tau, kind=kind, cs=cs, real=real)
     O.cavity(L, (lid, 0.0, 0.0), lid_axis=2)
+    # the GPU fields are float32: give the oracle the same (rounded) wall velocity
+    L.vel[...] = L.vel.astype(np.float32).astype(real)
     return L
 
 

@@ -1,0 +1,109 @@
+"""DEM on the GPU vs the oracle: bit-exact broad phase, trajectories within FP64 rounding."""
+import numpy as np
+import pytest
+import torch
+
+from dem_cases import gpu_solver, make_config, oracle_solver, random_packing, read_p4p
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("fixture", ["vf0.35.p4p", "shear_5.p4p", "sill_5.p4p", "hybridcf.p4p", "random"])
+@pytest.mark.parametrize("probe27", [False, True])
+def test_broadphase_bit_exact(fixture, probe27):
+    """Candidate pairs of the HIP broad phase == the restated reference set, as sorted sets."""
+    from src.dem3d.bpcd import BPCD
+    if fixture == "random":
+        rng = np.random.default_rng(5)
+        pos = rng.uniform(0.0, 1.0, (20000, 3))
+        rad = rng.uniform(0.002, 0.006, 20000)
+    else:
+        a = read_p4p(fixture)
+        rad, pos = a[:, 2], a[:, 4:7]
+    dmin = pos.min(0) - rad.max()
+    dmax = pos.max(0) + rad.max()
+    cell = 2.0 * rad.max() if probe27 else 4.0 * rad.max()
+    bp = BPCD.create(len(pos), rad.max(), dmin, dmax, cell_size=cell, probe27=probe27)
+    assert bp.table_size == O.hash_table_size(dmin, dmax, cell)
+    got = bp.detect_collision(torch.tensor(pos, dtype=torch.float64, device="cuda")).cpu().numpy()
+    got = got[np.lexsort((got[:, 1], got[:, 0]))]
+    ref = O.broadphase(pos, dmin, cell, bp.table_size, probe27=probe27)
+    print(f"{fixture} probe27={probe27}: {len(ref)} candidate pairs", flush=True)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+
+
+def test_broadphase_empty_and_single():
+    from src.dem3d.bpcd import BPCD
+    bp = BPCD.create(1, 0.1, [0, 0, 0], [1, 1, 1])
+    got = bp.detect_collision(torch.tensor([[0.5, 0.5, 0.5]], dtype=torch.float64, device="cuda"))
+    assert got.shape == (0, 2)
+
+
+def _run_pair(model, path, n=120, steps=400, periodic=(False, False, False), seed=1):
+    rng = np.random.default_rng(seed)
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    cfg = make_config(model=model, box=box, dt=5e-5, periodic=periodic, kn=2e3, kt=2e3, zeta=0.3)
+    rad, mass, pos, vel = random_packing(n, 0.045, box, rng, v=0.5, polydisperse=0.3)
+    gpu = gpu_solver(cfg, rad, mass, pos, vel, path=path)
+    orc = oracle_solver(cfg, rad, mass, pos, vel)
+    for _ in range(steps // 50):
+        gpu.run_substeps(50)
+        orc.step(50)
+    torch.cuda.synchronize()
+    return gpu, orc
+
+
+@pytest.mark.parametrize("model", ["hertz", "linear"])
+@pytest.mark.parametrize("path", [1, 2])
+def test_dem_trajectories_match_oracle(model, path):
+    gpu, orc = _run_pair(model, path)
+    pos = gpu.gf.position.to_numpy()
+    vel = gpu.gf.velocity.to_numpy()
+    om = gpu.gf.omega.to_numpy()
+    assert np.abs(pos - orc["pos"]).max() < 1e-9
+    assert np.abs(vel - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
+    assert np.abs(om - orc["omega"]).max() < 1e-6 * max(1.0, np.abs(orc["omega"]).max())
+    assert np.allclose(np.linalg.norm(gpu.gf.quaternion.to_numpy(), axis=1), 1.0, atol=1e-12)
+    # contact history: same active (i, j) set
+    K = gpu.K
+    act = gpu._cf_active.cpu().numpy().astype(bool)
+    gi = np.repeat(np.arange(gpu.n), K)[act]
+    gj = gpu._cf_j.cpu().numpy()[act]
+    oact = orc["cf_active"].astype(bool)
+    oi = np.repeat(np.arange(orc.n), K)[oact]
+    oj = orc["cf_j"][oact]
+    assert sorted(zip(gi.tolist(), gj.tolist())) == sorted(zip(oi.tolist(), oj.tolist()))
+    assert gpu.status()["slot_overflow"] == 0
+
+
+def test_dem_periodic_axis():
+    gpu, orc = _run_pair("hertz", 0, periodic=(True, False, False), seed=4)
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+
+
+def test_dem_small_and_general_paths_agree():
+    a, _ = _run_pair("hertz", 1, seed=9)
+    b, _ = _run_pair("hertz", 2, seed=9)
+    assert np.abs(a.gf.position.to_numpy() - b.gf.position.to_numpy()).max() < 1e-10
+
+
+def test_dem_p4p_loader_and_writer(tmp_path):
+    """init_particle_fields reads a reference packing; save_single writes p4p / p4c."""
+    from src.dem3d.demsolver import DEMSolver
+    import os
+    from dem_cases import GOLDEN
+    a = read_p4p("shear_5.p4p")
+    lo, hi = a[:, 4:7].min(0) - 0.01, a[:, 4:7].max(0) + 0.01
+    cfg = make_config(box=(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]), dt=1e-6)
+    s = DEMSolver(cfg)
+    s.init_particle_fields(os.path.join(GOLDEN, "shear_5.p4p"), lo, hi)
+    s.set_contact_model("hertz")
+    assert s.n == len(a)
+    assert np.allclose(s.gf.position.to_numpy(), a[:, 4:7])
+    s.run_simulation()
+    with open(tmp_path / "o.p4p", "w") as p4p, open(tmp_path / "o.p4c", "w") as p4c:
+        s.save_single(p4p, p4c, 0.0)
+    txt = (tmp_path / "o.p4p").read_text().splitlines()
+    assert txt[0].startswith("TIMESTEP") and int(txt[1].split()[1]) == len(a)
