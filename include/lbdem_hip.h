@@ -73,7 +73,8 @@ typedef struct lbdem_lattice {
   double rho0;         /* shift density of the stored populations */
   double force[3];     /* body force in lattice units: adds rho*w_q*(c_q.F)/cs^2 */
   int32_t kernel;      /* collide-stream variant: 0 automatic, 1 one cell per thread,
-                          2 row-tiled (one workgroup per (i, j) row, aligned accesses) */
+                          2 row-tiled (one workgroup per (i, j) row, aligned accesses; two cells
+                          per thread for 16-bit storage), 3 row-tiled, two cells per thread */
   int32_t reserved;
 } lbdem_lattice;
 
@@ -188,6 +189,39 @@ int lbdem_psc_decode(const lbdem_lattice *lat, const uint32_t *key, const float 
                      float *volfrac, float *weight, float *velsolid, void *stream);
 
 /* ------------------------------------------------------------------------------------ */
+/* unresolved coupling (src/unlbdem)                                                      */
+/* ------------------------------------------------------------------------------------ */
+/* EqIBM spreading: EqIMBlattice3D.grains2lattice (src/unlbdem/eqlattice.py:104-194) with the
+ * mirror kernel (:468-528) and threedelta (:451-462).  Zeroes and accumulates volfrac,
+ * velsum (3n AoS), wsum; clamps volfrac >= 1 to 0.99 (counted in status[0] if non-NULL);
+ * velsolid = velsum / volfrac.  Also writes rho, vel (AoS) of the current post-stream state
+ * (step args: f, flags, t) at covered FLUID cells: the fluid velocity that compute_weight
+ * (:200-238, evaluated inside lbdem_lbm_step) and lattice2grains read. */
+int lbdem_ibm_spread(const lbdem_lattice *lat, const lbdem_step_args *a, const lbdem_grains *g,
+                     const lbdem_units *u, float *volfrac, float *velsum, float *wsum,
+                     float *velsolid, float *rho, float *vel, int32_t *status, void *stream);
+
+/* EqIBM interpolation + Tenneti drag into g->force_fluid: EqIMBlattice3D.lattice2grains and
+ * compute_drag_force (src/unlbdem/eqlattice.py:309-444). */
+int lbdem_ibm_drag(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                   const lbdem_units *u, const float *volfrac, const float *vel, void *stream);
+
+/* Tenneti collision weight per covered cell (EqIMBlattice3D.compute_weight,
+ * src/unlbdem/eqlattice.py:200-238) for inspection; the step kernel evaluates it inline. */
+int lbdem_ibm_weight(const lbdem_lattice *lat, const uint8_t *flags, const float *volfrac,
+                     const float *velsolid, const float *vel, float *weight, void *stream);
+
+/* VANS spreading: Unresolvedlattice3D.force2lattice (src/unlbdem/vanslattice.py:489-557), or
+ * grains2lattice (:135-196, volume only) when hforce is NULL. */
+int lbdem_vans_spread(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                      const lbdem_units *u, float *volfrac, float *hforce, void *stream);
+
+/* VANS drag: Unresolvedlattice3D.compute_hydrodynamic_force + compute_drag_force
+ * (src/unlbdem/vanslattice.py:446-482, 563-651) into g->force_fluid. */
+int lbdem_vans_drag(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                    const lbdem_units *u, const float *volfrac, const float *vel, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
 /* DEM (src/dem3d)                                                                        */
 /* ------------------------------------------------------------------------------------ */
 /* material row layout: density, E, nu, friction, restitution, k_n, k_t, damping_n, damping_t */
@@ -257,6 +291,38 @@ int lbdem_dem_broadphase(const lbdem_dem_params *P, const double *pos, const lbd
  * 2 general grid-wide path. */
 int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbdem_dem_work *w,
                   int32_t nsub, int32_t path, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* differentiable PSC channel (forward + hand-written adjoint)                            */
+/* ------------------------------------------------------------------------------------ */
+/* No counterpart in the reference (it has no adjoint).  The forward model is the PSC
+ * collision with fixed grains (src/lbdem/psclattice3d.py:240-261) plus the body-force term of
+ * ForcedLattice3D (src/lbm3d/lbm_foecelover.py:53-68) on the fluid part, (1 - B) rho w (c.F)/cs^2,
+ * and bounce-back / periodic streaming (src/lbm3d/lbm_solver3d.py:203-224).  Two-array layout,
+ * unshifted populations f[q*ncell + c], FP32 or FP64. */
+typedef struct lbdem_diff_args {
+  int32_t nx, ny, nz;
+  int32_t precision;        /* 32 or 64: element type of every state / adjoint / eps array */
+  const uint8_t *flags;     /* ncell flag bytes (ACTIVE, WALL) */
+  const void *eps;          /* ncell solid fraction per cell */
+  double tau;               /* relaxation time */
+  double force[3];          /* body force, lattice units */
+  int32_t plane;            /* outlet plane index along x */
+  int32_t reserved;
+} lbdem_diff_args;
+
+/* f <- rest state (rho = 1, u = 0) at active cells */
+int lbdem_diff_init(const lbdem_diff_args *d, void *f, void *stream);
+/* f_out = S(C(f_in)) */
+int lbdem_diff_step(const lbdem_diff_args *d, const void *f_in, void *f_out, void *stream);
+/* lam_out = C'(f_n)^T S^T lam_in; g_eps[c] += dL/d eps_c; g_red[0] += dL/dtau,
+ * g_red[1..3] += dL/dF (FP64) */
+int lbdem_diff_adjoint_step(const lbdem_diff_args *d, const void *f_n, const void *lam_in, void *lam_out,
+                            void *g_eps, double *g_red, void *stream);
+/* u_x on the plane i = plane (ny*nz) */
+int lbdem_diff_outlet(const lbdem_diff_args *d, const void *f, void *u_plane, void *stream);
+/* lam = dL/df of the state f given dL/du_x on the plane (g_u, ny*nz); zero off the plane */
+int lbdem_diff_seed(const lbdem_diff_args *d, const void *f, const void *g_u, void *lam, void *stream);
 
 #ifdef __cplusplus
 }

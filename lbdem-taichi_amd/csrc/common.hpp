@@ -48,35 +48,59 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // ---------------------------------------------------------------------------------------
 // storage types: load/store with conversion to the compute type
 // ---------------------------------------------------------------------------------------
+// Streaming loads/stores of the population arrays: non-temporal unless built with
+// -DLBDEM_TEMPORAL_POPULATIONS.
+template <typename R>
+__device__ __forceinline__ R lbdem_ld_raw(const R *p) {
+#ifdef LBDEM_TEMPORAL_POPULATIONS
+  return *p;
+#else
+  return __builtin_nontemporal_load(p);
+#endif
+}
+template <typename R>
+__device__ __forceinline__ void lbdem_st_raw(R *p, R v) {
+#ifdef LBDEM_TEMPORAL_POPULATIONS
+  *p = v;
+#else
+  __builtin_nontemporal_store(v, p);
+#endif
+}
+
+// cvt: stored element -> compute type; pack: compute type -> stored element
 template <typename S> struct Store;
 template <> struct Store<float> {
   using T = float;
-  static __device__ __forceinline__ float ld(const float *p) { return __builtin_nontemporal_load(p); }
-  static __device__ __forceinline__ void st(float *p, float v) { __builtin_nontemporal_store(v, p); }
+  static __device__ __forceinline__ float cvt(float e) { return e; }
+  static __device__ __forceinline__ float pack(float v) { return v; }
+  static __device__ __forceinline__ float ld(const float *p) { return lbdem_ld_raw(p); }
+  static __device__ __forceinline__ void st(float *p, float v) { lbdem_st_raw(p, v); }
 };
 template <> struct Store<double> {
   using T = double;
-  static __device__ __forceinline__ double ld(const double *p) { return __builtin_nontemporal_load(p); }
-  static __device__ __forceinline__ void st(double *p, double v) { __builtin_nontemporal_store(v, p); }
+  static __device__ __forceinline__ double cvt(double e) { return e; }
+  static __device__ __forceinline__ double pack(double v) { return v; }
+  static __device__ __forceinline__ double ld(const double *p) { return lbdem_ld_raw(p); }
+  static __device__ __forceinline__ void st(double *p, double v) { lbdem_st_raw(p, v); }
 };
 template <> struct Store<_Float16> {
   using T = float;
-  static __device__ __forceinline__ float ld(const _Float16 *p) {
-    return (float)__builtin_nontemporal_load(p);
-  }
-  static __device__ __forceinline__ void st(_Float16 *p, float v) {
-    __builtin_nontemporal_store((_Float16)v, p);
-  }
+  static __device__ __forceinline__ float cvt(_Float16 e) { return (float)e; }
+  static __device__ __forceinline__ _Float16 pack(float v) { return (_Float16)v; }
+  static __device__ __forceinline__ float ld(const _Float16 *p) { return (float)lbdem_ld_raw(p); }
+  static __device__ __forceinline__ void st(_Float16 *p, float v) { lbdem_st_raw(p, (_Float16)v); }
 };
 template <> struct Store<__bf16> {
   using T = float;
+  static __device__ __forceinline__ float cvt(__bf16 e) {
+    return __uint_as_float(((uint32_t)__builtin_bit_cast(uint16_t, e)) << 16);
+  }
+  static __device__ __forceinline__ __bf16 pack(float v) { return (__bf16)v; }  // RNE (v_cvt_pk_bf16_f32)
   static __device__ __forceinline__ float ld(const __bf16 *p) {
-    const uint16_t b = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(p));
-    return __uint_as_float(((uint32_t)b) << 16);
+    return cvt(__builtin_bit_cast(__bf16, lbdem_ld_raw(reinterpret_cast<const uint16_t *>(p))));
   }
   static __device__ __forceinline__ void st(__bf16 *p, float v) {
-    const __bf16 h = (__bf16)v;  // round to nearest even (v_cvt_pk_bf16_f32)
-    __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h), reinterpret_cast<uint16_t *>(p));
+    lbdem_st_raw(reinterpret_cast<uint16_t *>(p), __builtin_bit_cast(uint16_t, pack(v)));
   }
 };
 

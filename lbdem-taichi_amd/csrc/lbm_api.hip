@@ -145,19 +145,24 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
   LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_VANS ||
                     (a->volfrac && a->hforce && a->volfrac_prev && a->hforce_prev && a->nu_lu > 0),
                 "lbdem_lbm_step: VANS needs volfrac, hforce, volfrac_prev, hforce_prev, nu_lu > 0");
-  // row-tiled variant: whole rows per workgroup, LDS staging of the z shift (<= 64 KiB)
-  const int V = t->pencil_v;
+  // row-tiled variants: whole rows per workgroup, LDS staging of the z shift (<= 64 KiB).
+  // kernel 2: one cell per thread (two for 16-bit storage); kernel 3: two cells per thread.
+  const bool half = lat->storage == LBDEM_STORE_F16 || lat->storage == LBDEM_STORE_BF16;
+  int kern = lat->kernel;
+  LBDEM_REQUIRE(kern >= 0 && kern <= 3, "lbdem_lbm_step: bad kernel variant %d", kern);
+  if (kern == 0) kern = 2;
+  const int V = (kern == 3 || half) ? 2 : 1;
   const int tsize = lat->storage == LBDEM_STORE_F64 ? 8 : 4;
   const int block = ((lat->nz / V + 63) / 64) * 64;
   const size_t lds = p.t_odd ? (size_t)10 * block * V * (tsize + 1) : 0;
   const bool pencil_ok = (lat->nz % V == 0) && block <= 1024 && lds <= 65536;
-  LBDEM_REQUIRE(lat->kernel != 2 || pencil_ok, "lbdem_lbm_step: row-tiled kernel needs nz %% %d == 0 and a row "
+  LBDEM_REQUIRE(lat->kernel < 2 || pencil_ok, "lbdem_lbm_step: row-tiled kernel needs nz %% %d == 0 and a row "
                 "of at most %d cells", V, 1024 * V);
-  if (t->pencil[coll][lat->coupling][p.t_odd] && (lat->kernel == 2 || (lat->kernel == 0 && pencil_ok))) {
+  KernelFn kp = (V == 2) ? t->pencil2[coll][lat->coupling][p.t_odd] : t->pencil[coll][lat->coupling][p.t_odd];
+  if (kern >= 2 && pencil_ok && kp) {
     void *args[] = {&p};
-    hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(t->pencil[coll][lat->coupling][p.t_odd]),
-                                   dim3((unsigned)(lat->nx * lat->ny)), dim3((unsigned)block), args, lds,
-                                   as_stream(stream));
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(kp), dim3((unsigned)(lat->nx * lat->ny)),
+                                   dim3((unsigned)block), args, lds, as_stream(stream));
     if (e != hipSuccess) {
       set_error("lbdem_lbm_step: launch failed: %s", hipGetErrorString(e));
       return LBDEM_E_HIP;

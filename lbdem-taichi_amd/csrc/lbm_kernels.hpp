@@ -541,7 +541,12 @@ constexpr int zslot() {
 }
 constexpr int kNZ = 10;
 
-// V consecutive elements (V = 1, or V = 2 for 16-bit storage) as one access
+// V consecutive elements (V = 1 or 2) as one access of V * sizeof(S) bytes
+template <int BYTES> struct Raw;
+template <> struct Raw<4> { using type = uint32_t; };
+template <> struct Raw<8> { using type = uint64_t; };
+template <> struct Raw<16> { using type = __attribute__((ext_vector_type(2))) uint64_t; };
+
 template <typename S, int V>
 struct Vec {
   using T = typename Store<S>::T;
@@ -549,25 +554,24 @@ struct Vec {
     if constexpr (V == 1) {
       v[0] = Store<S>::ld(p);
     } else {
-      static_assert(sizeof(S) == 2 && V == 2, "pairs of 16-bit values only");
-      const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
-      S a, b;
-      __builtin_memcpy(&a, &w, 2);
-      const uint32_t hi = w >> 16;
-      __builtin_memcpy(&b, &hi, 2);
-      v[0] = Store<S>::ld(&a);
-      v[1] = Store<S>::ld(&b);
+      static_assert(V == 2, "one or two elements");
+      using R = typename Raw<2 * sizeof(S)>::type;
+      const R w = lbdem_ld_raw(reinterpret_cast<const R *>(p));
+      S e[2];
+      __builtin_memcpy(e, &w, sizeof(e));
+      v[0] = Store<S>::cvt(e[0]);
+      v[1] = Store<S>::cvt(e[1]);
     }
   }
   static __device__ __forceinline__ void st(S *p, const T (&v)[V]) {
     if constexpr (V == 1) {
       Store<S>::st(p, v[0]);
     } else {
-      S a = (S)v[0], b = (S)v[1];
-      uint16_t ua, ub;
-      __builtin_memcpy(&ua, &a, 2);
-      __builtin_memcpy(&ub, &b, 2);
-      __builtin_nontemporal_store((uint32_t)ua | ((uint32_t)ub << 16), reinterpret_cast<uint32_t *>(p));
+      using R = typename Raw<2 * sizeof(S)>::type;
+      S e[2] = {Store<S>::pack(v[0]), Store<S>::pack(v[1])};
+      R w;
+      __builtin_memcpy(&w, e, sizeof(e));
+      lbdem_st_raw(reinterpret_cast<R *>(p), w);
     }
   }
 };

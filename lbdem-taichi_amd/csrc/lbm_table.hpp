@@ -8,8 +8,8 @@ using KernelFn = void (*)(const KParams);
 
 struct LbmKernels {
   KernelFn step[3][4][2];  // [collision][coupling][odd]; nullptr = unsupported combination
-  KernelFn pencil[3][4][2];  // row-tiled variant of step (same semantics)
-  int pencil_v;              // cells per thread of the row-tiled variant
+  KernelFn pencil[3][4][2];   // row-tiled variant of step, one cell per thread
+  KernelFn pencil2[3][4][2];  // row-tiled variant, two cells per thread
   KernelFn init[4];        // [coupling]
   KernelFn macros[4];      // [coupling]
   KernelFn read_f;
@@ -31,37 +31,53 @@ const LbmKernels &lbm_kernels_f64();
     static const LbmKernels t = [] {                                                            \
       LbmKernels k{};                                                                           \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false>;               \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, (sizeof(S) == 2 ? 2 : 1)>;               \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 1>;               \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 2>;               \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true>;                \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, (sizeof(S) == 2 ? 2 : 1)>;                \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 1>;                \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 2>;                \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false>; \
-      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, (sizeof(S) == 2 ? 2 : 1)>; \
+      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 1>; \
+      k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 2>; \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true>;  \
-      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, (sizeof(S) == 2 ? 2 : 1)>;  \
+      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 1>;  \
+      k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 2>;  \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false>;   \
-      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, (sizeof(S) == 2 ? 2 : 1)>;   \
+      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 1>;   \
+      k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 2>;   \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true>;    \
-      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, (sizeof(S) == 2 ? 2 : 1)>;    \
+      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 1>;    \
+      k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 2>;    \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false>;                 \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, (sizeof(S) == 2 ? 2 : 1)>;                 \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 1>;                 \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 2>;                 \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true>;                  \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, (sizeof(S) == 2 ? 2 : 1)>;                  \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 1>;                  \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 2>;                  \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false>;             \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, (sizeof(S) == 2 ? 2 : 1)>;             \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 1>;             \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 2>;             \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true>;              \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, (sizeof(S) == 2 ? 2 : 1)>;              \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 1>;              \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 2>;              \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false>; \
-      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, (sizeof(S) == 2 ? 2 : 1)>; \
+      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 1>; \
+      k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 2>; \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true>;  \
-      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, (sizeof(S) == 2 ? 2 : 1)>;  \
+      k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 1>;  \
+      k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 2>;  \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false>;   \
-      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, (sizeof(S) == 2 ? 2 : 1)>;   \
+      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 1>;   \
+      k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 2>;   \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true>;    \
-      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, (sizeof(S) == 2 ? 2 : 1)>;    \
+      k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 1>;    \
+      k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 2>;    \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false>;               \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, (sizeof(S) == 2 ? 2 : 1)>;               \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 1>;               \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 2>;               \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true>;                \
-      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, (sizeof(S) == 2 ? 2 : 1)>;                \
+      k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 1>;                \
+      k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 2>;                \
       k.init[LBDEM_COUPLE_NONE] = k_init<S, LBDEM_COUPLE_NONE>;                                 \
       k.init[LBDEM_COUPLE_PSC] = k_init<S, LBDEM_COUPLE_NONE>;                                  \
       k.init[LBDEM_COUPLE_EQIBM] = k_init<S, LBDEM_COUPLE_NONE>;                                \
@@ -70,7 +86,6 @@ const LbmKernels &lbm_kernels_f64();
       k.macros[LBDEM_COUPLE_PSC] = k_macros<S, LBDEM_COUPLE_NONE>;                              \
       k.macros[LBDEM_COUPLE_EQIBM] = k_macros<S, LBDEM_COUPLE_NONE>;                            \
       k.macros[LBDEM_COUPLE_VANS] = k_macros<S, LBDEM_COUPLE_VANS>;                             \
-      k.pencil_v = sizeof(S) == 2 ? 2 : 1;                                                     \
       k.read_f = k_read_f<S>;                                                                   \
       k.write_f = k_write_f<S>;                                                                 \
       k.reduce_partial = k_reduce_partial<S>;                                                   \

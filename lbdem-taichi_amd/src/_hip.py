@@ -9,7 +9,7 @@ import os
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "liblbdem_hip.so"))
+LIB_PATH = os.environ.get("LBDEM_HIP_LIB") or os.path.normpath(os.path.join(_HERE, "..", "liblbdem_hip.so"))
 
 STORE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
 STORE_DTYPE = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}

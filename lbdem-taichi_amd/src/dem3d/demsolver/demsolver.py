@@ -270,6 +270,7 @@ class DEMSolver:
 
     def set_contact_model(self, model_type: str = "hertz"):
         """Select the contact law; must match the config's contact model (demsolver.py:311-347)."""
+        self._cached = None
         cm = self.config.contact_model
         if model_type.lower() == "hertz":
             if not isinstance(cm, HertzContactConfig):
@@ -327,15 +328,24 @@ class DEMSolver:
             setattr(S, name, getattr(self, "_" + name).data_ptr())
         return S
 
+    def invalidate(self):
+        """Rebuild the kernel parameter block at the next step (after changing the config)."""
+        self._cached = None
+
     def run_substeps(self, nsub: int):
         """nsub DEM substeps (each one run_simulation of the reference) in one call."""
         if nsub <= 0 or self.n == 0:
             return
-        P = self._params()
-        S = self._state()
-        _, w = self.bpcd.workspace(self.n, self._pair_cap, self.device)
-        _hip.check(_hip.lib().lbdem_dem_run(ctypes.byref(P), ctypes.byref(S), ctypes.byref(w), int(nsub),
-                                            int(self.path), _hip.stream()), "lbdem_dem_run")
+        cached = getattr(self, "_cached", None)
+        if cached is None:
+            P = self._params()
+            S = self._state()
+            _, w = self.bpcd.workspace(self.n, self._pair_cap, self.device)
+            cached = self._cached = (P, S, w, ctypes.byref(P), ctypes.byref(S), ctypes.byref(w))
+        else:
+            self.gf.sync()
+        _hip.check(_hip.lib().lbdem_dem_run(cached[3], cached[4], cached[5], int(nsub), int(self.path),
+                                            _hip.stream()), "lbdem_dem_run")
         self.gf.touch()
 
     def run_simulation(self):

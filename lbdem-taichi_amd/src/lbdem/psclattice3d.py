@@ -97,12 +97,20 @@ class PSCLattice3D(BasicLattice3D):
         super().initialize()
         self.grains2lattice()
 
+    def _structs(self):
+        if getattr(self, "_gu", None) is None:
+            g = grains_struct(self.dem)
+            u = units_struct(self.unit, self.dem)
+            self._gu = (g, u, ctypes.byref(g), ctypes.byref(u))
+        else:
+            self.dem.gf.sync()
+        return self._gu[2], self._gu[3]
+
     def grains2lattice(self):
         """Map grain id, solid fraction, weight and velocity to the lattice (:84-182)."""
         self._prepare()
-        g = grains_struct(self.dem)
-        u = units_struct(self.unit, self.dem)
-        _hip.check(_hip.lib().lbdem_psc_map(self._lat_c, _hip.ptr(self._flags), ctypes.byref(g), ctypes.byref(u),
+        g, u = self._structs()
+        _hip.check(_hip.lib().lbdem_psc_map(self._lat_c, _hip.ptr(self._flags), g, u,
                                             _hip.ptr(self._key), _hip.ptr(self._snap), _hip.ptr(self._bbox),
                                             1 if self._mapped else 0, _hip.stream()), "lbdem_psc_map")
         self._mapped = True
@@ -110,9 +118,8 @@ class PSCLattice3D(BasicLattice3D):
 
     def lattice2grains(self):
         """Hydrodynamic force and torque on the grains (:316-358), FP64, physical units."""
-        g = grains_struct(self.dem)
-        u = units_struct(self.unit, self.dem)
-        _hip.check(_hip.lib().lbdem_psc_forces(self._lat_c, ctypes.byref(g), ctypes.byref(u), _hip.ptr(self._key),
+        g, u = self._structs()
+        _hip.check(_hip.lib().lbdem_psc_forces(self._lat_c, g, u, _hip.ptr(self._key),
                                                _hip.ptr(self._cell_force), _hip.stream()), "lbdem_psc_forces")
         self.dem.gf.force_fluid.touch()
         self.dem.gf.moment_fluid.touch()

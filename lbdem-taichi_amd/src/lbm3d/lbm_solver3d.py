@@ -218,10 +218,20 @@ class BasicLattice3D:
         if self._lag is not None:
             self.omega.touch()
 
+    # optional (start, end) event pairs recorded around every collide-stream launch
+    step_events = None
+
     def collide(self):
         """One fused collide + stream step (lbm_solver3d.py:176-250)."""
         self._prepare()
+        ev = self.step_events
+        if ev is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
         _hip.check(_hip.lib().lbdem_lbm_step(self._lat_c, self._args_c(), _hip.stream()), "lbdem_lbm_step")
+        if ev is not None:
+            e1.record()
+            ev.append((e0, e1))
         self.t += 1
         if self._lag is not None:
             self.omega.touch()

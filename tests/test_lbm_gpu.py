@@ -139,7 +139,7 @@ def test_periodic_shear_wave_decay_no_smagorinsky():
     old = BasicLattice3D.SmagorinskyConstant
     BasicLattice3D.SmagorinskyConstant = 0.0
     try:
-        n, tau, A, T1, T2 = 32, 0.8, 1e-3, 100, 500
+        n, tau, A, T1, T2 = 64, 0.8, 1e-3, 200, 1000
         L = BasicLattice3D(4, n, 4, 1.0 / tau, 1.0, 1.0, 1.0, storage="f64")
         y = np.arange(n)
         v = np.zeros((4, n, 4, 3))
@@ -152,8 +152,9 @@ def test_periodic_shear_wave_decay_no_smagorinsky():
         a2 = L.vel.to_numpy()[1, :, 1, 0] @ np.sin(2 * np.pi * y / n) * 2 / n
         nu = (tau - 0.5) / 3.0
         k = 2 * np.pi / n
-        # the decay rate between two times after the start-up transient of an equilibrium start
-        assert abs((a2 / a1) / np.exp(-nu * k * k * (T2 - T1)) - 1.0) < 1e-3
+        # the decay rate between two times after the start-up transient of an equilibrium start;
+        # BGK's O(k^2) dispersion error is ~1e-3 at this wavelength (4x that at n = 32)
+        assert abs((a2 / a1) / np.exp(-nu * k * k * (T2 - T1)) - 1.0) < 2e-3
     finally:
         BasicLattice3D.SmagorinskyConstant = old
 
@@ -179,13 +180,14 @@ def test_row_tiled_kernel_equals_per_cell_kernel(smag017, storage, steps):
     inner = (slice(1, -1),) * 3
     orc.vel[inner] = rng.uniform(-0.03, 0.03, size=orc.vel[inner].shape).astype(np.float32)
     out = []
-    for kernel in (1, 2):
+    for kernel in (1, 2, 3):
         L = gpu_from_oracle(BasicLattice3D, orc, storage=storage)
         L.kernel = kernel
         L.initialize()
         L.step(steps)
         out.append(L.f.to_numpy())
     assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0], out[2])
 
 
 def test_row_tiled_kernel_periodic_box(smag017):

@@ -1,0 +1,117 @@
+"""Unresolved couplings (EqIBM, VANS) on the GPU vs the oracle's restatement."""
+import numpy as np
+import pytest
+
+from dem_cases import gpu_solver, make_config
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DX, DT, RHO = 1e-3, 5e-4, 1000.0
+
+
+def _particles(n=70, N=20, seed=8):
+    """Sub-grid spheres (R ~ 0.3 lu): a dense cluster (solid fraction clamps), particles within
+    1.5 lu of the walls (mirror images), and scattered ones."""
+    rng = np.random.default_rng(seed)
+    lat = np.concatenate([
+        rng.normal([6.0, 6.0, 6.0], 0.6, (25, 3)),            # cluster
+        np.column_stack([rng.uniform(1.0, 2.4, 15), rng.uniform(2, N - 3, 15), rng.uniform(2, N - 3, 15)]),
+        rng.uniform(2.0, N - 3.0, (n - 40, 3)),
+    ])
+    pos = (lat - 0.5) * DX
+    rad = rng.uniform(0.25, 0.4, len(pos)) * DX
+    vel = rng.uniform(-0.02, 0.02, (len(pos), 3)) * DX / DT
+    mass = 2500.0 * 4 / 3 * np.pi * rad ** 3
+    return pos, vel, rad, mass
+
+
+def _walls(CT):
+    CT[0], CT[-1] = O.OBSTACLE, O.OBSTACLE
+    CT[:, 0], CT[:, -1] = O.OBSTACLE, O.OBSTACLE
+    CT[:, :, 0], CT[:, :, -1] = O.OBSTACLE, O.OBSTACLE
+
+
+def _setup(cls, kind, N=20, tau=0.6, cs=0.17):
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    pos, vel, rad, mass = _particles(N=N)
+    box = (0.0, (N - 1) * DX, 0.0, (N - 1) * DX, 0.0, (N - 1) * DX)
+    cfg = make_config(box=box, dt=DT / 10, g=(0, 0, 0))
+    dem = gpu_solver(cfg, rad, mass, pos, vel)
+    BasicLattice3D.SmagorinskyConstant = cs
+    L = cls(N, N, N, 1.0 / tau, DX, DT, RHO, dem)
+    orc = O.OracleLattice(N, N, N, 1.0 / tau, kind=kind, cs=cs)
+    _walls(orc.CT)
+    rng = np.random.default_rng(4)
+    inner = (slice(1, -1),) * 3
+    orc.vel[inner] = rng.uniform(-0.02, 0.02, orc.vel[inner].shape).astype(np.float32)
+    L.CT.from_numpy(orc.CT)
+    L.vel.from_numpy(orc.vel.astype(np.float32))
+    return L, orc, dem, pos, vel, rad
+
+
+def test_eqibm_matches_oracle():
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    from src.unlbdem.eqlattice import EqIMBlattice3D
+    old = BasicLattice3D.SmagorinskyConstant
+    try:
+        L, orc, dem, pos, vel, rad = _setup(EqIMBlattice3D, "eqibm")
+        g = np.column_stack([pos, vel, rad])
+        L.initialize_complete()
+        orc.initialize()
+        O.ibm_spread(orc, g, [0, 0, 0], DX, DT)
+        O.ibm_weight(orc, DX, DT, RHO)
+        assert (orc.volfrac == 0.99).sum() > 0, "the cluster must reach the solid-fraction clamp"
+        vf = L.volfrac.to_numpy()
+        assert np.abs(vf - orc.volfrac).max() < 2e-6
+        cov = orc.volfrac > 0
+        assert np.abs(L.velsolid.to_numpy()[cov] - orc.velsolid[cov]).max() < 1e-5 * np.abs(orc.velsolid).max()
+        assert np.abs(L.weight.to_numpy()[cov] - orc.weight[cov]).max() < 2e-4 * np.abs(orc.weight).max()
+        F = None
+        for _ in range(30):
+            orc.collide()
+            orc.stream()
+            O.ibm_spread(orc, g, [0, 0, 0], DX, DT)
+            O.ibm_weight(orc, DX, DT, RHO)
+            F = O.ibm_drag(orc, g, [0, 0, 0], DX, DT, RHO)
+            L.collide()
+            L.stream()
+            L.update_coupling()
+        fluid = orc.CT == O.FLUID
+        assert np.abs(L.vel.to_numpy()[fluid] - orc.vel[fluid]).max() < 2e-4 * np.abs(orc.vel[fluid]).max()
+        Fg = dem.gf.force_fluid.to_numpy()
+        assert np.abs(Fg - F).max() < 5e-4 * np.abs(F).max()
+    finally:
+        BasicLattice3D.SmagorinskyConstant = old
+
+
+def test_vans_matches_oracle():
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    from src.unlbdem.vanslattice import Unresolvedlattice3D
+    old = BasicLattice3D.SmagorinskyConstant
+    try:
+        L, orc, dem, pos, vel, rad = _setup(Unresolvedlattice3D, "vans")
+        gv = np.column_stack([pos, vel, rad])
+        L.initialize()
+        orc.initialize()
+        F = None
+        for _ in range(25):
+            # grains2lattice -> compute_hydrodynamic_force -> force2lattice -> collide -> stream
+            O.vans_spread(orc, np.column_stack([pos, rad, np.zeros_like(pos)]), [0, 0, 0], DX, DT, RHO,
+                          with_force=False)
+            F = O.vans_drag(orc, gv, [0, 0, 0], DX, DT, RHO)
+            O.vans_spread(orc, np.column_stack([pos, rad, F]), [0, 0, 0], DX, DT, RHO, with_force=True)
+            orc.collide()
+            orc.stream()
+            L.grains2lattice()
+            L.compute_hydrodynamic_force()
+            L.force2lattice()
+            L.collide()
+            L.stream()
+        fluid = orc.CT == O.FLUID
+        assert np.abs(L.volfrac.to_numpy() - orc.volfrac).max() < 2e-6
+        assert np.abs(L.vel.to_numpy()[fluid] - orc.vel[fluid]).max() < 2e-4 * np.abs(orc.vel[fluid]).max()
+        Fg = dem.gf.force_fluid.to_numpy()
+        assert np.abs(Fg - F).max() < 5e-4 * np.abs(F).max()
+    finally:
+        BasicLattice3D.SmagorinskyConstant = old
