@@ -151,6 +151,147 @@ def tgv_extra(storage, smag="fixed", n=256, steps=20):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": bpc}
 
 
+def precision_study(n=256, steps=20000, every=500, checkpoints=(1000, 5000, 10000, 20000), smag="lagged"):
+    """Taylor-Green vortex, Re = 1600, u0 = 0.05, periodic n^3, Smagorinsky Cs = 0.1 (lagged, the
+    reference's form).  Populations stored as shifted FP64 / FP32 / BF16 / FP16 (FP64 collision for
+    FP64 storage, FP32 otherwise).  Errors are measured against the FP64 run: kinetic-energy
+    history (FP64 sums) and the relative L2 velocity error at checkpoints."""
+    import numpy as np
+    import torch
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    u0, Re = 0.05, 1600.0
+    k = 2 * np.pi / n
+    nu = u0 / (k * Re)
+    tau = 3 * nu + 0.5
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    v = np.zeros((n, n, n, 3), np.float32)
+    v[..., 0] = u0 * np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z)
+    v[..., 1] = -u0 * np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z)
+    rho = (1.0 + 3.0 * u0 * u0 / 16.0 * (np.cos(2 * k * X) + np.cos(2 * k * Y)) * (np.cos(2 * k * Z) + 2)).astype(np.float32)
+    del X, Y, Z
+    ref_vel, ref_ke, out = {}, None, {}
+    for storage in ("f64", "f32", "bf16", "f16"):
+        L = BasicLattice3D(n, n, n, 1.0 / tau, 1.0, 1.0, 1.0, storage=storage, smagorinsky=smag)
+        L.vel.from_numpy(v)
+        L.rho.from_numpy(rho)
+        L.initialize()
+        ke, errs = [L.get_kinetic_energy()], {}
+        t0 = time.perf_counter()
+        for t in range(1, steps + 1):
+            L.collide()
+            if t % every == 0:
+                ke.append(L.get_kinetic_energy())
+            if t in checkpoints:
+                L._refresh_macros()
+                u = L.vel.dev.to(torch.float64)
+                if storage == "f64":
+                    ref_vel[t] = u.clone()
+                else:
+                    r = ref_vel[t]
+                    errs[t] = float(torch.linalg.vector_norm(u - r) / torch.linalg.vector_norm(r))
+        torch.cuda.synchronize()
+        ke = np.array(ke)
+        if storage == "f64":
+            ref_ke = ke
+        out[storage] = {
+            "wall_s": round(time.perf_counter() - t0, 2),
+            "ke_final_over_initial": float(ke[-1] / ke[0]),
+            "max_rel_ke_error": float(np.abs(ke / ref_ke - 1).max()),
+            "rel_l2_velocity_error": {str(t): e for t, e in errs.items()},
+        }
+        del L
+        torch.cuda.empty_cache()
+    return {"study": "precision", "lattice": n, "steps": steps, "Re": Re, "u0": u0, "tau": tau,
+            "smagorinsky": {"Cs": 0.1, "mode": smag}, "results": out}
+
+
+def adjoint_study(n=96, steps=2000, K=100, nsph=32, R=6.0, seed=7, n_fd_cells=16):
+    """Differentiable PSC channel: n^3, walls at y = 0 and y = n-1, periodic x and z, body force
+    F_x = 1e-6, 32 fixed spheres of radius 6 (seeded), tau = 0.6; loss = MSE of u_x on the
+    outlet plane x = n-1 against a seeded target (Poiseuille + 5 % noise); adjoint over 2000
+    steps with checkpoints every K = 100.  Reports the adjoint / forward time ratio (FP32) and
+    the gradients' relative errors against central differences and against the FP64 adjoint."""
+    import numpy as np
+    import torch
+    from src.adjoint import DiffPSCChannel, PSCRollout, sphere_fraction
+    from src.lbm3d.lbmutils import CellType
+    rng = np.random.default_rng(seed)
+    CT = np.full((n, n, n), CellType.FLUID, dtype=np.int32)
+    CT[:, 0, :] = CellType.OBSTACLE
+    CT[:, -1, :] = CellType.OBSTACLE
+    cen = []
+    while len(cen) < nsph:
+        c = np.array([rng.uniform(R + 1, n - R - 2), rng.uniform(R + 2, n - R - 3), rng.uniform(R + 1, n - R - 2)])
+        if all(np.linalg.norm(c - d) > 2 * R + 1 for d in cen):
+            cen.append(c)
+    eps0 = sphere_fraction((n, n, n), cen, R)
+    y = np.arange(n)
+    prof = np.clip((y - 0.5) * (n - 1.5 - y), 0, None)
+    prof = 0.8 * 1e-6 / (2 * (0.6 - 0.5) / 3) * prof[:, None] * np.ones((1, n))
+    target = prof * (1 + 0.05 * rng.standard_normal((n, n)))
+
+    def grads(prec):
+        dt = torch.float64 if prec == 64 else torch.float32
+        ch = DiffPSCChannel(CT, precision=prec, K=K)
+        F = torch.tensor([1e-6, 0.0, 0.0], dtype=dt, device="cuda", requires_grad=True)
+        tau = torch.tensor(0.6, dtype=dt, device="cuda", requires_grad=True)
+        e = torch.tensor(eps0, dtype=dt, device="cuda", requires_grad=True)
+        tg = torch.tensor(target, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u = PSCRollout.apply(F, tau, e, ch, steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss = ((u.double() - tg) ** 2).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return ch, tg, (F.grad.double().cpu().numpy(), float(tau.grad), e.grad.double().cpu().numpy()), \
+            float(loss), (t1 - t0) / steps, (t2 - t1) / steps
+
+    grads(32)  # warm-up (kernel load)
+    _, _, g32, _, fwd32, adj32 = grads(32)
+    ch, tg, g64, loss64, fwd64, adj64 = grads(64)
+
+    def L(Fx, tv, ev):
+        with torch.no_grad():
+            u = ch.forward((Fx, 0.0, 0.0), tv, torch.tensor(ev, dtype=torch.float64, device="cuda"), steps)
+            return float(((u - tg) ** 2).mean())
+
+    fd = []
+    h = 1e-4 * 1e-6
+    fd.append(("F_x", g64[0][0], (L(1e-6 + h, 0.6, eps0) - L(1e-6 - h, 0.6, eps0)) / (2 * h)))
+    h = 1e-4 * 0.6
+    fd.append(("tau", g64[1], (L(1e-6, 0.6 + h, eps0) - L(1e-6, 0.6 - h, eps0)) / (2 * h)))
+    fluid = np.argwhere((eps0 > 0) & (eps0 < 1))
+    pick = fluid[rng.choice(len(fluid), n_fd_cells // 2, replace=False)]
+    free = np.argwhere(eps0 == 0)
+    free = free[(free[:, 1] > 0) & (free[:, 1] < n - 1) & (free[:, 0] > n - 20)]
+    pick = list(pick) + list(free[rng.choice(len(free), n_fd_cells - len(pick), replace=False)])
+    for c in pick:
+        c = tuple(int(v) for v in c)
+        ep, em = eps0.copy(), eps0.copy()
+        ep[c] += 1e-4
+        em[c] -= 1e-4
+        fd.append((f"eps{c}", g64[2][c], (L(1e-6, 0.6, ep) - L(1e-6, 0.6, em)) / 2e-4))
+    rel = [abs(a - b) / max(abs(b), 1e-30) for _, a, b in fd]
+    gscale = max(abs(b) for _, _, b in fd)
+    absrel = [abs(a - b) / gscale for _, a, b in fd]
+    return {"study": "adjoint", "lattice": n, "steps": steps, "checkpoint_every": K, "spheres": nsph,
+            "loss": loss64,
+            "fp32": {"forward_ms_per_step": round(fwd32 * 1e3, 4), "adjoint_ms_per_step": round(adj32 * 1e3, 4),
+                     "adjoint_over_forward": round(adj32 / fwd32, 3)},
+            "fp64": {"forward_ms_per_step": round(fwd64 * 1e3, 4), "adjoint_ms_per_step": round(adj64 * 1e3, 4),
+                     "adjoint_over_forward": round(adj64 / fwd64, 3)},
+            "fd_check_fp64": {"max_rel_error": max(rel), "max_error_over_largest_gradient": max(absrel),
+                              "n_checked": len(fd),
+                              "entries": [(k, a, b) for k, a, b in fd]},
+            "fp32_vs_fp64": {"F": float(np.abs(g32[0] - g64[0]).max() / np.abs(g64[0]).max()),
+                             "tau": abs(g32[1] - g64[1]) / abs(g64[1]),
+                             "eps": float(np.abs(g32[2] - g64[2]).max() / np.abs(g64[2]).max())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +300,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--study", choices=["precision", "adjoint"], default=None,
+                    help="run a study instead of the benchmark and print its JSON summary")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 512 MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -169,6 +312,12 @@ def main():
 
     import torch
     torch.cuda.set_device(0)
+    if args.study == "precision":
+        print(json.dumps(precision_study()), flush=True)
+        return
+    if args.study == "adjoint":
+        print(json.dumps(adjoint_study()), flush=True)
+        return
     if args.calibrate_copy:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

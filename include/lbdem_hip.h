@@ -74,7 +74,10 @@ typedef struct lbdem_lattice {
   double force[3];     /* body force in lattice units: adds rho*w_q*(c_q.F)/cs^2 */
   int32_t kernel;      /* collide-stream variant: 0 automatic, 1 one cell per thread,
                           2 row-tiled (one workgroup per (i, j) row, aligned accesses; two cells
-                          per thread for 16-bit storage), 3 row-tiled, two cells per thread */
+                          per thread for 16-bit storage), 3 row-tiled, two cells per thread,
+                          4 / 5 / 6 / 7 wave-shift (aligned row accesses, z shift by lane
+                          shuffles), 1 / 2 / 4 / 8 cells per thread, nz a multiple of that;
+                          6 is not built for FP64 storage, 7 only for 16-bit storage */
   int32_t reserved;
 } lbdem_lattice;
 
@@ -270,8 +273,8 @@ typedef struct lbdem_dem_work {
   int32_t *count, *offset, *current; /* T each */
   int32_t *tile_sum;                 /* ceil(T / 1024) */
   int32_t *hash, *pid;               /* n each */
-  int32_t *probe;                    /* 27n (small-system path) */
-  int32_t *pairs;                    /* 2 * pair_cap candidate pairs (i < j) */
+  int32_t *pairs;                    /* 2 * pair_cap candidate pairs (i < j); the small-system
+                                        path lists the new contacts of a substep here */
   int64_t pair_cap;
   unsigned long long *npairs;        /* 1: candidate count (may exceed pair_cap) */
   int32_t *need_new;                 /* pair_cap */

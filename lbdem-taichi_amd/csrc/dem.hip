@@ -6,9 +6,8 @@
 //    reference's Morton hash and counting sort, two-phase contact resolution, walls,
 //    integration, slot compaction);
 //  - the small-system path (n <= kSmallN): one workgroup runs many substeps in a single
-//    launch, with barriers between phases, and finds candidate pairs by comparing each
-//    particle's cell hash with the probe-cell hashes of the other, which yields exactly the
-//    set the hash-table probe yields.
+//    launch, with barriers between phases, and finds candidate pairs through bucket lists in
+//    LDS keyed by the same cell hash, which yields exactly the set the hash-table probe yields.
 //
 // FP64 throughout.  The float -> cell arithmetic is compiled without contraction
 // (-ffp-contract=off) so cell keys match the CPU restatement bit for bit.
@@ -640,66 +639,65 @@ __global__ __launch_bounds__(256) void k_dem_compact(const lbdem_dem_params P, c
 // --------------------------------------------------------------------------------------
 // small-system path: one workgroup, `nsub` substeps per launch
 // --------------------------------------------------------------------------------------
+// The whole kernel is one workgroup, so __syncthreads() (a workgroup-scope fence plus barrier)
+// orders the phases' global-memory accesses; no device-scope fence (L2 write-back) is needed.
+// Candidate pairs come from bucket lists kept in LDS: particle j is pushed onto the list of
+// slot (hash_j & (kSmallSlots - 1)); particle i walks the lists of its distinct probe hashes and
+// keeps the j > i whose own hash equals the probed one.  That is the candidate rule of the hash
+// table (j's bucket is one of i's probed buckets) without the 2^20..2^22-entry table.
+constexpr int kSmallSlots = 4096;
+
 __global__ __launch_bounds__(1024) void k_dem_small(const lbdem_dem_params P, const lbdem_dem_state S, int nsub,
-                                                    int32_t *probe, int32_t *cell_hash, int32_t *newlist,
-                                                    int newcap) {
-  const int n = P.n;
-  const int np = P.probe27 ? 27 : 8;
+                                                    int32_t *newlist, int newcap) {
+  __shared__ int32_t head[kSmallSlots];
+  __shared__ int32_t next[kSmallN];
+  __shared__ int32_t chash[kSmallN];
   __shared__ int nnew;
+  const int n = P.n;
+  for (int t = threadIdx.x; t < kSmallSlots; t += 1024) head[t] = -1;
+  __syncthreads();
   for (int sub = 0; sub < nsub; ++sub) {
-    // clear_state + probe hashes of every particle
+    // clear_state, own cell hash, push onto the bucket list
     for (int i = threadIdx.x; i < n; i += 1024) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) S.force[3 * i + a] = S.moment[3 * i + a] = 0.0;
-      int32_t h[27];
-      probe_hashes(P, S.pos + 3 * i, h);
-      for (int k = 0; k < np; ++k) probe[i * 27 + k] = h[k];
       int32_t c[3];
       cell_of(P, S.pos + 3 * i, c);
-      cell_hash[i] = hash3(c[0], c[1], c[2], P.T);
+      const int32_t h = hash3(c[0], c[1], c[2], P.T);
+      chash[i] = h;
+      next[i] = atomicExch(&head[h & (kSmallSlots - 1)], i);
     }
     if (threadIdx.x == 0) nnew = 0;
-    __threadfence();
     __syncthreads();
-    // candidate pairs: j's bucket is one of i's probed buckets; existing contacts first
-    const long long npair = (long long)n * (n - 1) / 2;
-    for (long long t = threadIdx.x; t < npair; t += 1024) {
-      // t -> (i, j), i < j, row-major over the upper triangle
-      int i = (int)((2.0 * n - 1.0 - sqrt((2.0 * n - 1.0) * (2.0 * n - 1.0) - 8.0 * (double)t)) * 0.5);
-      long long row0 = (long long)i * (2LL * n - i - 1) / 2;
-      while (row0 > t) {
-        --i;
-        row0 = (long long)i * (2LL * n - i - 1) / 2;
-      }
-      while ((long long)(i + 1) * (2LL * n - i - 2) / 2 <= t) {
-        ++i;
-        row0 = (long long)i * (2LL * n - i - 1) / 2;
-      }
-      const int j = i + 1 + (int)(t - row0);
-      const int32_t hj = cell_hash[j];
-      bool cand = false;
-      for (int k = 0; k < np; ++k) cand |= (probe[i * 27 + k] == hj);
-      if (!cand) continue;
-      if (resolve_existing(P, S, i, j)) {
-        const int slot = atomicAdd(&nnew, 1);
-        if (slot < newcap) {
-          newlist[2 * slot] = i;
-          newlist[2 * slot + 1] = j;
+    // candidates of i; contacts already in a history slot are resolved here, new ones listed
+    for (int i = threadIdx.x; i < n; i += 1024) {
+      int32_t h[27];
+      const int nc = probe_hashes(P, S.pos + 3 * i, h);
+      for (int k = 0; k < nc; ++k) {
+        if (h[k] < 0) continue;
+        for (int j = head[h[k] & (kSmallSlots - 1)]; j >= 0; j = next[j]) {
+          if (j <= i || chash[j] != h[k]) continue;
+          if (resolve_existing(P, S, i, j)) {
+            const int slot = atomicAdd(&nnew, 1);
+            if (slot < newcap) {
+              newlist[2 * slot] = i;
+              newlist[2 * slot + 1] = j;
+            }
+          }
         }
       }
     }
-    __threadfence();
     __syncthreads();
     const int nn = min(nnew, newcap);
     if (threadIdx.x == 0 && nnew > newcap) atomicAdd(S.status + 0, nnew - newcap);
     for (int t = threadIdx.x; t < nn; t += 1024) resolve_new(P, S, newlist[2 * t], newlist[2 * t + 1]);
-    __threadfence();
+    for (int t = threadIdx.x; t < kSmallSlots; t += 1024) head[t] = -1;  // lists are rebuilt next substep
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 1024) walls_and_integrate(P, S, i);
-    __threadfence();
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 1024) compact_slots(P, S, i);
-    __threadfence();
+    // integration and slot compaction touch particle i's own data only
+    for (int i = threadIdx.x; i < n; i += 1024) {
+      walls_and_integrate(P, S, i);
+      compact_slots(P, S, i);
+    }
     __syncthreads();
   }
 }
@@ -761,10 +759,9 @@ int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbd
   const bool small = (path == 1) || (path == 0 && n <= kSmallN);
   if (small) {
     LBDEM_REQUIRE(n <= kSmallN, "lbdem_dem_run: the single-workgroup path takes at most %d particles", kSmallN);
-    LBDEM_REQUIRE(w->probe && w->hash && w->pairs && w->pair_cap > 0,
-                  "lbdem_dem_run: small path needs probe, hash and pairs");
+    LBDEM_REQUIRE(w->pairs && w->pair_cap > 0, "lbdem_dem_run: small path needs the pairs buffer");
     const int newcap = (int)min(w->pair_cap, (int64_t)0x7fffffff);
-    hipLaunchKernelGGL(k_dem_small, dim3(1), dim3(1024), 0, s, *P, *S, nsub, w->probe, w->hash, w->pairs, newcap);
+    hipLaunchKernelGGL(k_dem_small, dim3(1), dim3(1024), 0, s, *P, *S, nsub, w->pairs, newcap);
     LBDEM_CHECK_LAUNCH("lbdem_dem_run(small)");
     return LBDEM_OK;
   }

@@ -147,10 +147,28 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
                 "lbdem_lbm_step: VANS needs volfrac, hforce, volfrac_prev, hforce_prev, nu_lu > 0");
   // row-tiled variants: whole rows per workgroup, LDS staging of the z shift (<= 64 KiB).
   // kernel 2: one cell per thread (two for 16-bit storage); kernel 3: two cells per thread.
+  // wave-shift variants: 256-thread workgroups over row segments, z shift by lane shuffles.
+  // kernels 4, 5, 6, 7: 1, 2, 4, 8 cells per thread (at most 16 bytes per access).
   const bool half = lat->storage == LBDEM_STORE_F16 || lat->storage == LBDEM_STORE_BF16;
   int kern = lat->kernel;
-  LBDEM_REQUIRE(kern >= 0 && kern <= 3, "lbdem_lbm_step: bad kernel variant %d", kern);
+  LBDEM_REQUIRE(kern >= 0 && kern <= 7, "lbdem_lbm_step: bad kernel variant %d", kern);
   if (kern == 0) kern = 2;
+  if (kern >= 4) {  // wave-shift variants: any row length, nz % V == 0
+    const int Vw = 1 << (kern - 4);
+    LBDEM_REQUIRE(lat->nz % Vw == 0, "lbdem_lbm_step: %d cells per thread need nz %% %d == 0", Vw, Vw);
+    const KernelFn(*tab)[4][2] = kern == 4 ? t->wave1 : kern == 5 ? t->wave2 : kern == 6 ? t->wave4 : t->wave8;
+    KernelFn kw = tab[coll][lat->coupling][p.t_odd];
+    LBDEM_REQUIRE(kw, "lbdem_lbm_step: no wave-shift kernel with %d cells per thread for this storage", Vw);
+    const int bpr = (lat->nz + 256 * Vw - 1) / (256 * Vw);
+    void *args[] = {&p};
+    hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(kw), dim3((unsigned)(lat->nx * lat->ny * bpr)),
+                                   dim3(256), args, 0, as_stream(stream));
+    if (e != hipSuccess) {
+      set_error("lbdem_lbm_step: launch failed: %s", hipGetErrorString(e));
+      return LBDEM_E_HIP;
+    }
+    return LBDEM_OK;
+  }
   const int V = (kern == 3 || half) ? 2 : 1;
   const int tsize = lat->storage == LBDEM_STORE_F64 ? 8 : 4;
   const int block = ((lat->nz / V + 63) / 64) * 64;

@@ -550,25 +550,28 @@ template <> struct Raw<16> { using type = __attribute__((ext_vector_type(2))) ui
 template <typename S, int V>
 struct Vec {
   using T = typename Store<S>::T;
+  static constexpr int kBytes = V * (int)sizeof(S);
   static __device__ __forceinline__ void ld(const S *p, T (&v)[V]) {
     if constexpr (V == 1) {
       v[0] = Store<S>::ld(p);
     } else {
-      static_assert(V == 2, "one or two elements");
-      using R = typename Raw<2 * sizeof(S)>::type;
+      static_assert(kBytes == 4 || kBytes == 8 || kBytes == 16, "4, 8 or 16 bytes per access");
+      using R = typename Raw<kBytes>::type;
       const R w = lbdem_ld_raw(reinterpret_cast<const R *>(p));
-      S e[2];
+      S e[V];
       __builtin_memcpy(e, &w, sizeof(e));
-      v[0] = Store<S>::cvt(e[0]);
-      v[1] = Store<S>::cvt(e[1]);
+#pragma unroll
+      for (int m = 0; m < V; ++m) v[m] = Store<S>::cvt(e[m]);
     }
   }
   static __device__ __forceinline__ void st(S *p, const T (&v)[V]) {
     if constexpr (V == 1) {
       Store<S>::st(p, v[0]);
     } else {
-      using R = typename Raw<2 * sizeof(S)>::type;
-      S e[2] = {Store<S>::pack(v[0]), Store<S>::pack(v[1])};
+      using R = typename Raw<kBytes>::type;
+      S e[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) e[m] = Store<S>::pack(v[m]);
       R w;
       __builtin_memcpy(&w, e, sizeof(e));
       lbdem_st_raw(reinterpret_cast<R *>(p), w);
@@ -757,6 +760,219 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
         }
       });
     }
+  }
+}
+
+}  // namespace lbdem
+
+// ---------------------------------------------------------------------------------------
+// Wave-shift variant: 256 threads cover 256 V consecutive cells of one (i, j) row (a row
+// longer than that takes several workgroups).  All row accesses are aligned; the +-1 shift
+// along k of the odd step moves values between neighbouring lanes of a wavefront with a
+// shuffle, and only the element that crosses a wavefront boundary (or wraps around the row)
+// is loaded or stored on its own.  No LDS staging and no barriers.
+// ---------------------------------------------------------------------------------------
+namespace lbdem {
+
+__device__ __forceinline__ int wrapk(int k, int nz) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); }
+
+template <typename T>
+__device__ __forceinline__ T lane_shfl(T v, int src) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), src, 64);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __shfl(__builtin_bit_cast(int, v), src, 64));
+  }
+}
+
+template <typename S, int COLL, int COUP, bool ODD, int V>
+__global__ __launch_bounds__(256) void k_step_wave(const KParams p) {
+  using T = typename Store<S>::T;
+  const int nz = p.nz;
+  const int seg_cells = 256 * V;
+  const int bpr = (nz + seg_cells - 1) / seg_cells;
+  const int row = blockIdx.x / bpr;
+  const int seg = blockIdx.x - row * bpr;
+  const int i = row / p.ny, j = row - (row / p.ny) * p.ny;
+  const int lane = threadIdx.x & 63;
+  const int k0 = seg * seg_cells + threadIdx.x * V;            // first of this lane's V cells
+  const int wk0 = seg * seg_cells + (threadIdx.x & ~63) * V;   // first cell of the wavefront
+  const int wk1 = min(wk0 + 64 * V, nz) - 1;                   // last valid cell of the wavefront
+  const bool lane_in = k0 < nz;                                // nz % V == 0: all V cells valid
+  const int prv = (lane + 63) & 63, nxt = (lane + 1) & 63;
+  S *__restrict__ F = reinterpret_cast<S *>(p.f);
+  const size_t N = (size_t)p.ncell;
+  const int base = row * nz;
+
+  uint8_t fl[V];
+  bool act[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    fl[v] = lane_in ? p.flags[base + k0 + v] : (uint8_t)0;
+    act[v] = fl[v] & LBDEM_FLAG_ACTIVE;
+  }
+  T f[V][Q];
+
+  // ---------------------------------------------------------------- gather
+  if constexpr (!ODD) {
+    if (lane_in) {
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        T t[V];
+        Vec<S, V>::ld(F + q * N + base + k0, t);
+#pragma unroll
+        for (int v = 0; v < V; ++v) f[v][q] = t[v];
+      });
+    }
+  } else {
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      const S *src = F + kOPP[q] * N + (size_t)(wrapi(i - cx, p.nx) * p.ny + wrapi(j - cy, p.ny)) * nz;
+      T a[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) a[v] = T(0);
+      if (lane_in) Vec<S, V>::ld(src + k0, a);
+      if constexpr (cz == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) f[v][q] = a[v];
+      } else if constexpr (cz == 1) {
+        // cells k0 - 1 .. k0 + V - 2: the previous lane's last element, then own 0 .. V - 2
+        const T s = lane_shfl(a[V - 1], prv);
+        f[0][q] = (k0 - 1 < wk0) ? (lane_in ? Store<S>::ld(src + wrapk(k0 - 1, nz)) : T(0)) : s;
+#pragma unroll
+        for (int v = 1; v < V; ++v) f[v][q] = a[v - 1];
+      } else {
+        // cells k0 + 1 .. k0 + V: own 1 .. V - 1, then the next lane's first element
+        const T s = lane_shfl(a[0], nxt);
+#pragma unroll
+        for (int v = 0; v < V - 1; ++v) f[v][q] = a[v + 1];
+        f[V - 1][q] = (k0 + V > wk1) ? (lane_in ? Store<S>::ld(src + wrapk(k0 + V, nz)) : T(0)) : s;
+      }
+    });
+    if (lane_in) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (fl[v] & LBDEM_FLAG_NEAR) {  // populations bounced back at a wall last step
+          const int k = k0 + v;
+          static_for<1, Q>([&](auto qc) {
+            constexpr int q = qc;
+            const int n = nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
+            if (p.flags[n] & kMaskSolid) f[v][q] = Store<S>::ld(F + q * N + base + k);
+          });
+        }
+      }
+    }
+  }
+
+  // ---------------------------------------------------------------- collide
+  T rho[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    rho[v] = T(0);
+    if (act[v]) rho[v] = collide<COLL, COUP, T>(p, base + k0 + v, i, j, k0 + v, f[v]);
+  }
+
+  // ---------------------------------------------------------------- scatter
+  if constexpr (!ODD) {
+    if (lane_in) {
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        T t[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          t[v] = f[v][q];
+          if (q != 0 && (fl[v] & LBDEM_FLAG_NEAR)) {
+            const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
+            if (p.flags[m] & LBDEM_FLAG_MOVING) t[v] -= ladd_term<q, T>(p, m, rho[v]);
+          }
+        }
+        Vec<S, V>::st(F + kOPP[q] * N + base + k0, t);  // inactive cells' slots are never read
+      });
+    }
+  } else {
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      S *dst = F + q * N + (size_t)(wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny)) * nz;
+      bool ok[V];
+      T val[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        val[v] = f[v][q];
+        ok[v] = act[v];
+        if (q != 0 && act[v] && (fl[v] & LBDEM_FLAG_NEAR)) {
+          const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
+          const uint8_t fm = p.flags[m];
+          if (fm & kMaskSolid) {  // bounce back into the own opposite slot
+            T b = val[v];
+            if (fm & LBDEM_FLAG_MOVING) b -= ladd_term<q, T>(p, m, rho[v]);
+            Store<S>::st(F + kOPP[q] * N + base + k0 + v, b);
+            ok[v] = false;
+          }
+        }
+      }
+      // out[v]: value for dst[k0 + v], stored when oko[v]
+      T out[V];
+      bool oko[V];
+      if constexpr (cz == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          out[v] = val[v];
+          oko[v] = ok[v];
+        }
+      } else if constexpr (cz == 1) {
+        // positions k0 .. k0 + V - 1 receive cells k0 - 1 .. k0 + V - 2
+        const T s = lane_shfl(val[V - 1], prv);
+        const int so = __shfl((int)ok[V - 1], prv, 64);
+        out[0] = s;
+        oko[0] = lane_in && (k0 - 1 >= wk0) && so;
+#pragma unroll
+        for (int v = 1; v < V; ++v) {
+          out[v] = val[v - 1];
+          oko[v] = ok[v - 1];
+        }
+        // the last cell's value leaves this wavefront (or wraps): stored by its own lane
+        if (lane_in && ok[V - 1] && k0 + V > wk1) Store<S>::st(dst + wrapk(k0 + V, nz), val[V - 1]);
+      } else {
+        // positions k0 .. k0 + V - 1 receive cells k0 + 1 .. k0 + V
+        const T s = lane_shfl(val[0], nxt);
+        const int so = __shfl((int)ok[0], nxt, 64);
+#pragma unroll
+        for (int v = 0; v < V - 1; ++v) {
+          out[v] = val[v + 1];
+          oko[v] = ok[v + 1];
+        }
+        out[V - 1] = s;
+        oko[V - 1] = lane_in && (k0 + V <= wk1) && so;
+        if (lane_in && ok[0] && k0 - 1 < wk0) Store<S>::st(dst + wrapk(k0 - 1, nz), val[0]);
+      }
+      if (lane_in) {
+        bool all = true;
+#pragma unroll
+        for (int v = 0; v < V; ++v) all = all && oko[v];
+        if (all) {
+          Vec<S, V>::st(dst + k0, out);
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v)
+            if (oko[v]) Store<S>::st(dst + k0 + v, out[v]);
+        }
+      }
+    });
+  }
+}
+
+// wave-shift kernel with V cells per thread, or nullptr where one access would exceed 16 bytes
+template <typename S, int COLL, int COUP, bool ODD, int V>
+constexpr void (*wave_fn())(const KParams) {
+  if constexpr (V * sizeof(S) <= 16) {
+    return k_step_wave<S, COLL, COUP, ODD, V>;
+  } else {
+    return nullptr;
   }
 }
 

@@ -2,7 +2,7 @@
 // per-grain FP64 hydrodynamic force / torque reduction.  The weighted collision itself is
 // folded into the collide-stream kernel (coupling LBDEM_COUPLE_PSC).
 //
-// Mapping is grain-centric: one workgroup per grain walks the grain's bounding box and
+// Mapping is grain-centric: the workgroups of a grain walk the grain's bounding box and
 // publishes, per covered cell, a 32-bit key with atomicMax:
 //     bit 30      full coverage (dist <= r - sqrt(3)/2)
 //     bits 29..23 sub-cell count (1..125) of a partially covered cell
@@ -45,6 +45,20 @@ __device__ __forceinline__ void grain_box(const PscArgs &a, int g, double &xc, d
   b[5] = min((int)(zc + r + 2), a.nz);
 }
 
+// Each grain's box is shared by kSplit workgroups (blockIdx.y), so that a few hundred grains
+// still fill the chip.
+constexpr int kSplit = 8;
+
+// min { x >= 0 : sqrt(x) >= r } for the correctly rounded sqrt: sqrt(x) < r <=> x < bound,
+// which replaces one FP64 square root per sub-cell sample with a compare.
+__device__ __forceinline__ double sqrt_lt_bound(double r) {
+  if (!(r > 0.0)) return 0.0;
+  double t = r * r;
+  while (t > 0.0 && sqrt(t) >= r) t = nextafter(t, 0.0);
+  while (sqrt(t) < r) t = nextafter(t, HUGE_VAL);
+  return t;
+}
+
 // zero the keys written by the previous mapping (its boxes are in bbox)
 __global__ __launch_bounds__(256) void k_psc_clear(const PscArgs a) {
   const int g = blockIdx.x;
@@ -52,7 +66,7 @@ __global__ __launch_bounds__(256) void k_psc_clear(const PscArgs a) {
   const int ni = b[1] - b[0], nj = b[3] - b[2], nk = b[5] - b[4];
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
   const int tot = ni * nj * nk;
-  for (int t = threadIdx.x; t < tot; t += 256) {
+  for (int t = blockIdx.y * 256 + threadIdx.x; t < tot; t += 256 * kSplit) {
     const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
     a.key[(i * a.ny + j) * a.nz + k] = 0u;
   }
@@ -66,7 +80,8 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
   grain_box(a, g, xc, yc, zc, b);
   const double r = (a.rad[g] - 0.5 * a.dx) / a.dx;
   const double half_diag = 0.5 * sqrt(3.0);
-  if (threadIdx.x == 0) {
+  const double r2 = sqrt_lt_bound(r);
+  if (threadIdx.x == 0 && blockIdx.y == 0) {
 #pragma unroll
     for (int m = 0; m < 6; ++m) a.bbox[6 * g + m] = b[m];
     // snapshot for the collision: centre (lattice units), v*dt/dx, omega*dt
@@ -86,7 +101,7 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
   const int tot = ni * nj * nk;
   const uint32_t gid = (uint32_t)(kMaxGrain - g);
-  for (int t = threadIdx.x; t < tot; t += 256) {
+  for (int t = blockIdx.y * 256 + threadIdx.x; t < tot; t += 256 * kSplit) {
     const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
     const int c = (i * a.ny + j) * a.nz + k;
     if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) continue;
@@ -105,8 +120,7 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
           const double dyy = (yc - py) * (yc - py);
           for (int kk = 0; kk < 5; ++kk) {
             const double pz = (double)k - 0.4 + 0.2 * kk;
-            const double d2 = sqrt(dxx + dyy + (zc - pz) * (zc - pz));
-            cnt += (d2 < r) ? 1 : 0;
+            cnt += (dxx + dyy + (zc - pz) * (zc - pz) < r2) ? 1 : 0;  // sqrt(.) < r
           }
         }
       }
@@ -250,10 +264,10 @@ int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_gr
   a.bbox = bbox;
   if (g->n == 0) return LBDEM_OK;
   if (clear_previous) {
-    hipLaunchKernelGGL(k_psc_clear, dim3(g->n), dim3(256), 0, as_stream(stream), a);
+    hipLaunchKernelGGL(k_psc_clear, dim3(g->n, kSplit), dim3(256), 0, as_stream(stream), a);
     LBDEM_CHECK_LAUNCH("lbdem_psc_map(clear)");
   }
-  hipLaunchKernelGGL(k_psc_map, dim3(g->n), dim3(256), 0, as_stream(stream), a);
+  hipLaunchKernelGGL(k_psc_map, dim3(g->n, kSplit), dim3(256), 0, as_stream(stream), a);
   LBDEM_CHECK_LAUNCH("lbdem_psc_map");
   return LBDEM_OK;
 }

@@ -9,15 +9,26 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("fixture", ["vf0.35.p4p", "shear_5.p4p", "sill_5.p4p", "hybridcf.p4p", "random"])
+@pytest.mark.parametrize("fixture", ["vf0.35.p4p", "shear_5.p4p", "sill_5.p4p", "hybridcf.p4p", "random",
+                                     "bed200k"])
 @pytest.mark.parametrize("probe27", [False, True])
 def test_broadphase_bit_exact(fixture, probe27):
-    """Candidate pairs of the HIP broad phase == the restated reference set, as sorted sets."""
+    """Candidate pairs of the HIP broad phase == the restated reference set, as sorted sets.
+
+    bed200k is the unresolved workload's particle set: 200,000 spheres, log-normal diameters
+    (mean 0.5 lu, sigma_log 0.2, clipped below 1 lu) in a full-width 128 x 128 bed at porosity
+    ~0.6 (2.25 lu tall), lattice units; 17.7 M (octant) / 9.3 M (27-cell) candidate pairs."""
     from src.dem3d.bpcd import BPCD
     if fixture == "random":
         rng = np.random.default_rng(5)
         pos = rng.uniform(0.0, 1.0, (20000, 3))
         rad = rng.uniform(0.002, 0.006, 20000)
+    elif fixture == "bed200k":
+        rng = np.random.default_rng(6)
+        n = 200_000
+        d = np.minimum(np.exp(np.log(0.5) - 0.02 + 0.2 * rng.standard_normal(n)), 0.999)
+        rad = 0.5 * d
+        pos = np.column_stack([rng.uniform(0, 128, n), rng.uniform(0, 128, n), rng.uniform(0, 2.25, n)])
     else:
         a = read_p4p(fixture)
         rad, pos = a[:, 2], a[:, 4:7]

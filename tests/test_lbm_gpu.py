@@ -173,21 +173,51 @@ def test_populations_round_trip(smag017):
 @pytest.mark.parametrize("storage", ["f32", "bf16", "f16", "f64"])
 @pytest.mark.parametrize("steps", [6, 7])
 def test_row_tiled_kernel_equals_per_cell_kernel(smag017, storage, steps):
-    """The row-tiled variant (aligned accesses, LDS z-shift) computes bit-identical states."""
+    """The row-tiled (LDS z-shift) and wave-shift (lane-shuffle z-shift) variants compute
+    states bit-identical to the per-cell kernel."""
     from src.lbm3d.lbm_solver3d import BasicLattice3D
     orc = oracle_cavity(18)
     rng = np.random.default_rng(1)
     inner = (slice(1, -1),) * 3
     orc.vel[inner] = rng.uniform(-0.03, 0.03, size=orc.vel[inner].shape).astype(np.float32)
     out = []
-    for kernel in (1, 2, 3):
+    kernels = (1, 2, 3, 4, 5)  # nz = 18: the 4- and 8-cell variants are covered by the long rows
+    for kernel in kernels:
         L = gpu_from_oracle(BasicLattice3D, orc, storage=storage)
         L.kernel = kernel
         L.initialize()
         L.step(steps)
         out.append(L.f.to_numpy())
-    assert np.array_equal(out[0], out[1])
-    assert np.array_equal(out[0], out[2])
+    for m in range(1, len(kernels)):
+        assert np.array_equal(out[0], out[m]), f"kernel {kernels[m]}"
+
+
+@pytest.mark.parametrize("shape", [(4, 3, 600), (5, 4, 131), (3, 3, 64)])
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_kernel_variants_long_rows(smag017, shape, storage):
+    """Rows longer than one wavefront / workgroup, odd row lengths and scattered obstacles
+    (bounce-back inside a row and across wavefront boundaries), periodic otherwise."""
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    nx, ny, nz = shape
+    orc = O.OracleLattice(nx, ny, nz, 1.0 / 0.6, kind="basic", cs=0.17)
+    rng = np.random.default_rng(nz)
+    solid = rng.random(shape) < 0.04
+    orc.CT[solid] = O.OBSTACLE
+    orc.vel[...] = rng.uniform(-0.03, 0.03, size=orc.vel.shape).astype(np.float32)
+    orc.vel[solid] = 0.0
+    even = nz % 2 == 0
+    # 16-bit storage runs the row-tiled kernel two cells per thread (nz must be even)
+    kernels = [1, 4] + ([2] if even or storage == "f32" else []) + ([3, 5] if even else []) + \
+        ([6] if nz % 4 == 0 else []) + ([7] if nz % 8 == 0 and storage != "f32" else [])
+    out = {}
+    for kernel in kernels:
+        L = gpu_from_oracle(BasicLattice3D, orc, storage=storage)
+        L.kernel = kernel
+        L.initialize()
+        L.step(7)
+        out[kernel] = L.f.to_numpy()
+    for kernel in kernels[1:]:
+        assert np.array_equal(out[1], out[kernel]), f"kernel {kernel}"
 
 
 def test_row_tiled_kernel_periodic_box(smag017):
@@ -197,10 +227,14 @@ def test_row_tiled_kernel_periodic_box(smag017):
     orc = O.OracleLattice(n, n, n, 1.0 / 0.6, kind="basic", cs=0.17)
     rng = np.random.default_rng(3)
     orc.vel[...] = rng.uniform(-0.03, 0.03, size=orc.vel.shape).astype(np.float32)
-    gpu = gpu_from_oracle(BasicLattice3D, orc, storage="f64")
-    gpu.kernel = 2
+    gpus = []
+    for kernel in (2, 4, 5):
+        gpu = gpu_from_oracle(BasicLattice3D, orc, storage="f64")
+        gpu.kernel = kernel
+        gpu.initialize()
+        gpus.append((kernel, gpu))
     orc.initialize()
-    gpu.initialize()
     orc.step(9)
-    gpu.step(9)
-    assert rel_err(gpu.f.to_numpy(), orc.f) < 1e-12
+    for kernel, gpu in gpus:
+        gpu.step(9)
+        assert rel_err(gpu.f.to_numpy(), orc.f) < 1e-12, f"kernel {kernel}"

@@ -10,17 +10,25 @@ pytestmark = pytest.mark.gpu
 DX, DT, RHO = 1e-3, 5e-4, 1000.0
 
 
-def _particles(n=70, N=20, seed=8):
-    """Sub-grid spheres (R ~ 0.3 lu): a dense cluster (solid fraction clamps), particles within
-    1.5 lu of the walls (mirror images), and scattered ones."""
+def _particles(n=70, N=20, seed=8, cluster=False):
+    """Sub-grid spheres (R ~ 0.3 lu) within 1.5 lu of the walls (mirror images) and scattered
+    ones.  cluster=True adds 25 larger spheres (R ~ 0.48 lu) packed round one point so that the
+    spread solid fraction passes 1 and is clamped to 0.99.
+
+    The cluster is used for one mapping only: the reference's Tenneti weight is not bounded by 1
+    and a packed cluster drives the fluid unstable within ~20 steps (oracle and reference alike),
+    and the VANS force is spread with the sign of the particle force, a positive feedback that
+    the cluster also makes unstable; the trajectories use the scattered spheres."""
     rng = np.random.default_rng(seed)
-    lat = np.concatenate([
-        rng.normal([6.0, 6.0, 6.0], 0.6, (25, 3)),            # cluster
-        np.column_stack([rng.uniform(1.0, 2.4, 15), rng.uniform(2, N - 3, 15), rng.uniform(2, N - 3, 15)]),
-        rng.uniform(2.0, N - 3.0, (n - 40, 3)),
-    ])
+    parts = [np.column_stack([rng.uniform(1.0, 2.4, 15), rng.uniform(2, N - 3, 15), rng.uniform(2, N - 3, 15)]),
+             rng.uniform(2.0, N - 3.0, (n - 40, 3))]
+    radii = [rng.uniform(0.25, 0.4, n - 25)]
+    if cluster:
+        parts.insert(0, rng.normal([6.0, 6.0, 6.0], 0.25, (25, 3)))
+        radii.insert(0, rng.uniform(0.45, 0.5, 25))
+    lat = np.concatenate(parts)
     pos = (lat - 0.5) * DX
-    rad = rng.uniform(0.25, 0.4, len(pos)) * DX
+    rad = np.concatenate(radii) * DX
     vel = rng.uniform(-0.02, 0.02, (len(pos), 3)) * DX / DT
     mass = 2500.0 * 4 / 3 * np.pi * rad ** 3
     return pos, vel, rad, mass
@@ -32,9 +40,9 @@ def _walls(CT):
     CT[:, :, 0], CT[:, :, -1] = O.OBSTACLE, O.OBSTACLE
 
 
-def _setup(cls, kind, N=20, tau=0.6, cs=0.17):
+def _setup(cls, kind, N=20, tau=0.6, cs=0.17, cluster=False):
     from src.lbm3d.lbm_solver3d import BasicLattice3D
-    pos, vel, rad, mass = _particles(N=N)
+    pos, vel, rad, mass = _particles(N=N, cluster=cluster)
     box = (0.0, (N - 1) * DX, 0.0, (N - 1) * DX, 0.0, (N - 1) * DX)
     cfg = make_config(box=box, dt=DT / 10, g=(0, 0, 0))
     dem = gpu_solver(cfg, rad, mass, pos, vel)
@@ -61,7 +69,6 @@ def test_eqibm_matches_oracle():
         orc.initialize()
         O.ibm_spread(orc, g, [0, 0, 0], DX, DT)
         O.ibm_weight(orc, DX, DT, RHO)
-        assert (orc.volfrac == 0.99).sum() > 0, "the cluster must reach the solid-fraction clamp"
         vf = L.volfrac.to_numpy()
         assert np.abs(vf - orc.volfrac).max() < 2e-6
         cov = orc.volfrac > 0
@@ -79,6 +86,37 @@ def test_eqibm_matches_oracle():
             L.update_coupling()
         fluid = orc.CT == O.FLUID
         assert np.abs(L.vel.to_numpy()[fluid] - orc.vel[fluid]).max() < 2e-4 * np.abs(orc.vel[fluid]).max()
+        Fg = dem.gf.force_fluid.to_numpy()
+        assert np.abs(Fg - F).max() < 5e-4 * np.abs(F).max()
+    finally:
+        BasicLattice3D.SmagorinskyConstant = old
+
+
+def test_eqibm_clamped_cluster_mapping():
+    """grains2lattice + compute_weight with a packed cluster: the clamp to 0.99, the solid
+    velocity and the weights match the oracle, and the clamp counter equals the clamped cells."""
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    from src.unlbdem.eqlattice import EqIMBlattice3D
+    old = BasicLattice3D.SmagorinskyConstant
+    try:
+        L, orc, dem, pos, vel, rad = _setup(EqIMBlattice3D, "eqibm", cluster=True)
+        g = np.column_stack([pos, vel, rad])
+        L.initialize_complete()
+        orc.initialize()
+        O.ibm_spread(orc, g, [0, 0, 0], DX, DT)
+        O.ibm_weight(orc, DX, DT, RHO)
+        clamped = int((orc.volfrac == 0.99).sum())
+        assert clamped > 0, "the cluster must reach the solid-fraction clamp"
+        assert L.clamp_count() == clamped
+        vf = L.volfrac.to_numpy()
+        assert np.abs(vf - orc.volfrac).max() < 2e-6
+        cov = orc.volfrac > 0
+        assert np.abs(L.velsolid.to_numpy()[cov] - orc.velsolid[cov]).max() < 1e-5 * np.abs(orc.velsolid).max()
+        # per cell: the weight of a clamped cell is ~1e3 and sensitive to FP32 rounding of 0.99
+        W, Wo = L.weight.to_numpy()[cov], orc.weight[cov]
+        assert np.all(np.abs(W - Wo) <= 1e-3 * np.abs(Wo) + 1e-6)
+        F = O.ibm_drag(orc, g, [0, 0, 0], DX, DT, RHO)
+        L.update_coupling()
         Fg = dem.gf.force_fluid.to_numpy()
         assert np.abs(Fg - F).max() < 5e-4 * np.abs(F).max()
     finally:
@@ -109,6 +147,7 @@ def test_vans_matches_oracle():
             L.collide()
             L.stream()
         fluid = orc.CT == O.FLUID
+        assert np.abs(orc.vel[fluid]).max() < 0.05, "the oracle trajectory must stay stable"
         assert np.abs(L.volfrac.to_numpy() - orc.volfrac).max() < 2e-6
         assert np.abs(L.vel.to_numpy()[fluid] - orc.vel[fluid]).max() < 2e-4 * np.abs(orc.vel[fluid]).max()
         Fg = dem.gf.force_fluid.to_numpy()
