@@ -194,6 +194,7 @@ def precision_study(n=256, steps=20000, every=500, checkpoints=(1000, 5000, 1000
         ke = np.array(ke)
         if storage == "f64":
             ref_ke = ke
+        print(f"precision study: {storage} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         out[storage] = {
             "wall_s": round(time.perf_counter() - t0, 2),
             "ke_final_over_initial": float(ke[-1] / ke[0]),
@@ -204,6 +205,79 @@ def precision_study(n=256, steps=20000, every=500, checkpoints=(1000, 5000, 1000
         torch.cuda.empty_cache()
     return {"study": "precision", "lattice": n, "steps": steps, "Re": Re, "u0": u0, "tau": tau,
             "smagorinsky": {"Cs": 0.1, "mode": smag}, "results": out}
+
+
+def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=10):
+    """Unresolved EqIBM LBM-DEM at its full single-GPU shape: 128^3 lattice (walls on all faces),
+    200,000 spheres with log-normal diameters (mean 0.5 lu, sigma_log 0.2, clipped below 1 lu)
+    seeded uniformly in a 64 x 64 x 9 lu bed on the floor (solid fraction ~0.4, overlaps allowed),
+    density ratio 2.5, gravity -1e-5 lu, linear contacts (k_n = k_t = 1, damping ratio 0.3,
+    friction 0.5), broad phase with cells one maximum diameter wide and the 27-cell probe, 10 DEM
+    substeps per lattice step, FP32 populations.  A step is: DEM substeps (general path),
+    collide-stream with the Tenneti weights, update_coupling (spreading, finalize, drag).
+    Reports per-phase times (HIP events) and the step's MLUPS."""
+    import numpy as np
+    import torch
+    from src.dem3d.demconfig import DEMSolverConfig, DomainBounds, LinearContactConfig
+    from src.dem3d.demsolver import DEMSolver
+    from src.unlbdem.eqlattice import EqIMBlattice3D
+    rng = np.random.default_rng(seed)
+    d = np.minimum(np.exp(np.log(0.5) - 0.02 + 0.2 * rng.standard_normal(n)), 0.999)
+    rad = 0.5 * d
+    L = np.array([N - 2.0, N - 2.0, N - 2.0])
+    lo = np.array([(L[0] - 64) / 2, (L[1] - 64) / 2, 0.0]) + 0.5
+    pos = lo + rng.random((n, 3)) * np.array([64.0, 64.0, 9.0])
+    mass = 2.5 * 4.0 / 3.0 * np.pi * rad ** 3
+    cm = LinearContactConfig(stiffness_normal=1.0, stiffness_tangential=1.0, damping_normal=0.3,
+                             damping_tangential=0.3, pp_friction=0.5, pw_friction=0.5)
+    cfg = DEMSolverConfig(domain=DomainBounds(0.0, L[0], 0.0, L[1], 0.0, L[2]), dt=1.0 / subcycles,
+                          gravity=(0.0, 0.0, -1e-5), contact_model=cm)
+    cfg.set_particle_properties(elastic_modulus=1.0, poisson_ratio=0.3, max_coordinate_number=16)
+    cfg.set_wall_properties(elastic_modulus=1.0, poisson_ratio=0.3, density=2.5)
+    dem = DEMSolver(cfg)
+    dem.init_particle_arrays(radius=rad, mass=mass, position=pos, cell_size=float(d.max()), probe27=True)
+    dem.set_contact_model("linear")
+    lat = EqIMBlattice3D(N, N, N, 1.0 / 0.6, 1.0, 1.0, 1.0, dem)
+    lat.CT.from_numpy(walls_ct(N, N, N))
+    lat.initialize_complete()
+
+    phases = ("dem", "collide_stream", "update_coupling")
+    ev = {k: [] for k in phases}
+
+    def step(record):
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if record else None
+        if record:
+            marks[0].record()
+        dem.run_substeps(subcycles)
+        if record:
+            marks[1].record()
+        lat.collide()
+        lat.stream()
+        if record:
+            marks[2].record()
+        lat.update_coupling()
+        if record:
+            marks[3].record()
+            for m, k in enumerate(phases):
+                ev[k].append((marks[m], marks[m + 1]))
+
+    for _ in range(warmup):
+        step(False)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step(True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    per = {k: round(sum(a.elapsed_time(b) for a, b in v) / len(v), 4) for k, v in ev.items()}
+    st = dem.status()
+    return {"study": "unresolved", "lattice": N, "particles": n, "steps": steps, "subcycles": subcycles,
+            "ms_per_step": round(ms, 4), "mlups": round(N ** 3 / ms / 1e3, 1), "phase_ms": per,
+            "dem_substep_ms": round(per["dem"] / subcycles, 4),
+            "dem_particle_substeps_per_s": round(n * subcycles / (per["dem"] * 1e-3), 1),
+            "dem_status": st, "eqibm_clamped_cells": lat.clamp_count()}
 
 
 def adjoint_study(n=96, steps=2000, K=100, nsph=32, R=6.0, seed=7, n_fd_cells=16):
@@ -300,7 +374,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--study", choices=["precision", "adjoint"], default=None,
+    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved"], default=None,
                     help="run a study instead of the benchmark and print its JSON summary")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 512 MiB three times (reference kernel for HBM byte counters)")
@@ -317,6 +391,9 @@ def main():
         return
     if args.study == "adjoint":
         print(json.dumps(adjoint_study()), flush=True)
+        return
+    if args.study == "unresolved":
+        print(json.dumps(unresolved_study()), flush=True)
         return
     if args.calibrate_copy:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 1
+#define LBDEM_ABI_VERSION 2
 
 /* return codes */
 #define LBDEM_OK 0
@@ -58,7 +58,7 @@ extern "C" {
 
 /* per-cell flag byte */
 #define LBDEM_FLAG_ACTIVE 0x01 /* collided and streamed (not OBSTACLE / VEL_LADD) */
-#define LBDEM_FLAG_NEAR 0x02   /* has a wall neighbour or lies on the domain edge (slow path) */
+#define LBDEM_FLAG_NEAR 0x02   /* has a wall neighbour (set by lbdem_lbm_prepare_flags) */
 #define LBDEM_FLAG_WALL 0x04   /* bounce-back wall (CellType.OBSTACLE) */
 #define LBDEM_FLAG_MOVING 0x08 /* moving bounce-back wall (CellType.VEL_LADD), velocity in vel[] */
 #define LBDEM_FLAG_FLUID 0x10  /* rho,u are refreshed after streaming (CellType.FLUID) */
@@ -72,12 +72,12 @@ typedef struct lbdem_lattice {
   double cs_smag;      /* Smagorinsky constant */
   double rho0;         /* shift density of the stored populations */
   double force[3];     /* body force in lattice units: adds rho*w_q*(c_q.F)/cs^2 */
-  int32_t kernel;      /* collide-stream variant: 0 automatic, 1 one cell per thread,
-                          2 row-tiled (one workgroup per (i, j) row, aligned accesses; two cells
-                          per thread for 16-bit storage), 3 row-tiled, two cells per thread,
-                          4 / 5 / 6 / 7 wave-shift (aligned row accesses, z shift by lane
-                          shuffles), 1 / 2 / 4 / 8 cells per thread, nz a multiple of that;
-                          6 is not built for FP64 storage, 7 only for 16-bit storage */
+  int32_t kernel;      /* collide-stream variant: 0 automatic (2 for FP32 storage, 3 for FP64
+                          and 16-bit storage, 1 where the row does not fit), 1 one cell per thread,
+                          2 / 3 / 4 row-tiled (one workgroup per (i, j) row, aligned accesses,
+                          z shift of odd steps through LDS) with 1 / 2 / 4 cells per thread;
+                          nz must be a multiple of that and at most 1024 times it; 4 is not
+                          built for FP64 storage */
   int32_t reserved;
 } lbdem_lattice;
 
@@ -85,6 +85,7 @@ typedef struct lbdem_lattice {
 typedef struct lbdem_step_args {
   void *f;                     /* 19*ncell populations, storage type */
   const uint8_t *flags;        /* ncell flag bytes */
+  const uint32_t *nmask;       /* ncell wall-neighbour masks from lbdem_lbm_prepare_flags */
   const float *vel;            /* 3*ncell (AoS): wall velocity read at MOVING cells */
   void *lag;                   /* 5*ncell (SoA rho,ux,uy,uz,omega) for SMAG_LAGGED; float,
                                   double with LBDEM_STORE_F64 */
@@ -104,6 +105,14 @@ typedef struct lbdem_step_args {
 
 int lbdem_abi_version(void);
 const char *lbdem_last_error(void);
+
+/* Derived flag data, on the device, from the ACTIVE / WALL / MOVING bits the caller sets: the
+ * NEAR bit of every cell (active with a wall among its 18 periodic neighbours) and the
+ * wall-neighbour mask of the NEAR cells (bit q, 1..18: x + c_q is a wall; bit 31: some
+ * neighbour is a moving wall; 0 elsewhere).  Call after every change of the flags.  It replaces
+ * the per-direction cell-type tests of BasicLattice3D.stream (src/lbm3d/lbm_solver3d.py:216-236)
+ * with one load per near-wall cell. */
+int lbdem_lbm_prepare_flags(const lbdem_lattice *lat, uint8_t *flags, uint32_t *nmask, void *stream);
 
 /* f <- feq(rho, vel) at active cells, natural layout (t = 0); lag state initialised.
  * Replaces BasicLattice3D.initialize (src/lbm3d/lbm_solver3d.py:157-171), also

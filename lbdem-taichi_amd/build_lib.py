@@ -16,18 +16,23 @@ ARCH = os.environ.get("LBDEM_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 generation.  -ffp-contract=off keeps the float
-# arithmetic that must match the CPU restatement bit for bit (broad-phase cell keys) free of
-# fused multiply-adds; kernels that want FMAs write them explicitly.
+# arithmetic that must match the CPU restatement bit for bit (broad-phase cell keys, PSC
+# sub-cell coverage) free of fused multiply-adds.  The collide-stream translation units contract
+# within each source expression (-ffp-contract=on): the collision is HBM- and issue-bound, the
+# FMAs cut its instruction count by a fifth, and every kernel variant inlines the same collision
+# source, so the variants stay bit-identical to each other.
 FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+CONTRACT = {"lbm_f32.hip", "lbm_f16.hip", "lbm_bf16.hip", "lbm_f64.hip"}
 
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    contract = "-ffp-contract=on" if os.path.basename(src) in CONTRACT else "-ffp-contract=off"
+    cmd = [HIPCC, *FLAGS, contract, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

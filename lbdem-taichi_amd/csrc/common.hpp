@@ -112,6 +112,7 @@ struct KParams {
   int ncell;
   void *f;
   const uint8_t *flags;
+  const uint32_t *nmask;  // wall-neighbour bits of NEAR cells (bit q: x + c_q is a wall)
   const float *vel;
   void *lag;  // float, or double with FP64 storage
   const uint32_t *psc_key;
@@ -136,6 +137,8 @@ struct KParams {
   const void *in;
   double *work;
 };
+
+constexpr uint32_t kNbrMoving = 1u << 31;  // nmask: some neighbour is a moving wall
 
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? n - 1 : (v >= n ? 0 : v); }
 

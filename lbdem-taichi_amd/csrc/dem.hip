@@ -645,20 +645,58 @@ __global__ __launch_bounds__(256) void k_dem_compact(const lbdem_dem_params P, c
 // slot (hash_j & (kSmallSlots - 1)); particle i walks the lists of its distinct probe hashes and
 // keeps the j > i whose own hash equals the probed one.  That is the candidate rule of the hash
 // table (j's bucket is one of i's probed buckets) without the 2^20..2^22-entry table.
+// With RESIDENT (n <= kResidentN, one thread per particle) the per-particle kinematic state
+// lives in LDS for the whole launch: the state view's pointers are redirected to LDS copies (flat addressing), so the
+// contact, wall and integration code is shared with the other paths; contact history stays in
+// global memory.
 constexpr int kSmallSlots = 4096;
+constexpr int kResidentN = 256;
 
-__global__ __launch_bounds__(1024) void k_dem_small(const lbdem_dem_params P, const lbdem_dem_state S, int nsub,
+template <bool RESIDENT, int NT>
+__global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, const lbdem_dem_state G, int nsub,
                                                     int32_t *newlist, int newcap) {
   __shared__ int32_t head[kSmallSlots];
   __shared__ int32_t next[kSmallN];
   __shared__ int32_t chash[kSmallN];
   __shared__ int nnew;
   const int n = P.n;
-  for (int t = threadIdx.x; t < kSmallSlots; t += 1024) head[t] = -1;
+  lbdem_dem_state S = G;
+  // 3-vectors and scalars copied to LDS: pos, vel, acc, force, omega, omega_dot, moment, quat(4),
+  // mass, radius, cfn
+  constexpr int kRes = RESIDENT ? kResidentN : 1;
+  __shared__ double r_vec[7][3 * kRes];
+  __shared__ double r_quat[4 * kRes];
+  __shared__ double r_mass[kRes], r_rad[kRes];
+  __shared__ int32_t r_cfn[kRes];
+  if constexpr (RESIDENT) {
+    double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
+    for (int t = threadIdx.x; t < 3 * n; t += NT) {
+#pragma unroll
+      for (int a = 0; a < 7; ++a) r_vec[a][t] = gv[a][t];
+    }
+    for (int t = threadIdx.x; t < 4 * n; t += NT) r_quat[t] = G.quat[t];
+    for (int t = threadIdx.x; t < n; t += NT) {
+      r_mass[t] = G.mass[t];
+      r_rad[t] = G.radius[t];
+      r_cfn[t] = G.cfn[t];
+    }
+    S.pos = r_vec[0];
+    S.vel = r_vec[1];
+    S.acc = r_vec[2];
+    S.force = r_vec[3];
+    S.omega = r_vec[4];
+    S.omega_dot = r_vec[5];
+    S.moment = r_vec[6];
+    S.quat = r_quat;
+    S.mass = r_mass;
+    S.radius = r_rad;
+    S.cfn = r_cfn;
+  }
+  for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;
   __syncthreads();
   for (int sub = 0; sub < nsub; ++sub) {
     // clear_state, own cell hash, push onto the bucket list
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int i = threadIdx.x; i < n; i += NT) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) S.force[3 * i + a] = S.moment[3 * i + a] = 0.0;
       int32_t c[3];
@@ -670,7 +708,7 @@ __global__ __launch_bounds__(1024) void k_dem_small(const lbdem_dem_params P, co
     if (threadIdx.x == 0) nnew = 0;
     __syncthreads();
     // candidates of i; contacts already in a history slot are resolved here, new ones listed
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int i = threadIdx.x; i < n; i += NT) {
       int32_t h[27];
       const int nc = probe_hashes(P, S.pos + 3 * i, h);
       for (int k = 0; k < nc; ++k) {
@@ -690,15 +728,24 @@ __global__ __launch_bounds__(1024) void k_dem_small(const lbdem_dem_params P, co
     __syncthreads();
     const int nn = min(nnew, newcap);
     if (threadIdx.x == 0 && nnew > newcap) atomicAdd(S.status + 0, nnew - newcap);
-    for (int t = threadIdx.x; t < nn; t += 1024) resolve_new(P, S, newlist[2 * t], newlist[2 * t + 1]);
-    for (int t = threadIdx.x; t < kSmallSlots; t += 1024) head[t] = -1;  // lists are rebuilt next substep
+    for (int t = threadIdx.x; t < nn; t += NT) resolve_new(P, S, newlist[2 * t], newlist[2 * t + 1]);
+    for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;  // lists are rebuilt next substep
     __syncthreads();
     // integration and slot compaction touch particle i's own data only
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int i = threadIdx.x; i < n; i += NT) {
       walls_and_integrate(P, S, i);
       compact_slots(P, S, i);
     }
     __syncthreads();
+  }
+  if constexpr (RESIDENT) {
+    double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
+    for (int t = threadIdx.x; t < 3 * n; t += NT) {
+#pragma unroll
+      for (int a = 0; a < 7; ++a) gv[a][t] = r_vec[a][t];
+    }
+    for (int t = threadIdx.x; t < 4 * n; t += NT) G.quat[t] = r_quat[t];
+    for (int t = threadIdx.x; t < n; t += NT) G.cfn[t] = r_cfn[t];
   }
 }
 
@@ -761,7 +808,11 @@ int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbd
     LBDEM_REQUIRE(n <= kSmallN, "lbdem_dem_run: the single-workgroup path takes at most %d particles", kSmallN);
     LBDEM_REQUIRE(w->pairs && w->pair_cap > 0, "lbdem_dem_run: small path needs the pairs buffer");
     const int newcap = (int)min(w->pair_cap, (int64_t)0x7fffffff);
-    hipLaunchKernelGGL(k_dem_small, dim3(1), dim3(1024), 0, s, *P, *S, nsub, w->pairs, newcap);
+    if (n <= kResidentN)
+      hipLaunchKernelGGL((k_dem_small<true, kResidentN>), dim3(1), dim3(kResidentN), 0, s, *P, *S, nsub, w->pairs,
+                         newcap);
+    else
+      hipLaunchKernelGGL((k_dem_small<false, 512>), dim3(1), dim3(512), 0, s, *P, *S, nsub, w->pairs, newcap);
     LBDEM_CHECK_LAUNCH("lbdem_dem_run(small)");
     return LBDEM_OK;
   }

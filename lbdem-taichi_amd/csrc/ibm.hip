@@ -387,9 +387,10 @@ int lbdem_ibm_spread(const lbdem_lattice *lat, const lbdem_step_args *st, const 
   kp.ncell = (int)n;
   kp.f = st->f;
   kp.flags = st->flags;
+  kp.nmask = st->nmask;
   kp.rho0 = lat->rho0;
   kp.t_odd = (int)(st->t & 1);
-  LBDEM_REQUIRE(st->f, "lbdem_ibm_spread: populations are required");
+  LBDEM_REQUIRE(st->f && st->nmask, "lbdem_ibm_spread: populations and nmask are required");
   void *args[] = {&a, &kp};
   const void *fn = nullptr;
   switch (lat->storage) {

@@ -36,7 +36,7 @@ static int fill_params(const lbdem_lattice *lat, const lbdem_step_args *a, KPara
   LBDEM_REQUIRE(n < (int64_t(1) << 31), "%s: %lld cells exceed 2^31", who, (long long)n);
   LBDEM_REQUIRE(lat->coupling >= 0 && lat->coupling < 4, "%s: bad coupling %d", who, lat->coupling);
   LBDEM_REQUIRE(lat->collision >= 0 && lat->collision < 3, "%s: bad collision %d", who, lat->collision);
-  LBDEM_REQUIRE(a->f && a->flags, "%s: f and flags are required", who);
+  LBDEM_REQUIRE(a->f && a->flags && a->nmask, "%s: f, flags and nmask are required", who);
   LBDEM_REQUIRE(lat->omega0 > 0.0, "%s: omega0 must be positive", who);
   p = KParams{};
   p.nx = lat->nx;
@@ -45,6 +45,7 @@ static int fill_params(const lbdem_lattice *lat, const lbdem_step_args *a, KPara
   p.ncell = (int)n;
   p.f = a->f;
   p.flags = a->flags;
+  p.nmask = a->nmask;
   p.vel = a->vel;
   p.lag = a->lag;
   p.psc_key = a->psc_key;
@@ -101,6 +102,29 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *work, int nb
   if (threadIdx.x < 5) out[threadIdx.x] = sh[0][threadIdx.x];
 }
 
+// NEAR bit and wall-neighbour mask of every cell (bit q: x + c_q is a wall; kNbrMoving: a
+// moving one).  Each thread rewrites only its own flag byte and reads only the neighbours'
+// WALL / MOVING bits, which this kernel does not change.
+__global__ __launch_bounds__(256) void k_prepare_flags(const KParams p, uint8_t *flags, uint32_t *nmask) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= p.ncell) return;
+  const uint8_t fl = flags[c];
+  uint32_t m = 0u;
+  if (fl & LBDEM_FLAG_ACTIVE) {
+    int i, j, k;
+    cell_ijk(p, c, i, j, k);
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+      const uint8_t fn = flags[nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q])];
+      if (fn & (LBDEM_FLAG_WALL | LBDEM_FLAG_MOVING)) m |= 1u << q;
+      if (fn & LBDEM_FLAG_MOVING) m |= kNbrMoving;
+    }
+  }
+  const uint8_t near = m ? LBDEM_FLAG_NEAR : 0;
+  if ((fl & LBDEM_FLAG_NEAR) != near) flags[c] = (uint8_t)((fl & ~LBDEM_FLAG_NEAR) | near);
+  nmask[c] = m;
+}
+
 }  // namespace lbdem
 
 using namespace lbdem;
@@ -108,6 +132,22 @@ using namespace lbdem;
 extern "C" {
 
 int lbdem_abi_version(void) { return LBDEM_ABI_VERSION; }
+
+int lbdem_lbm_prepare_flags(const lbdem_lattice *lat, uint8_t *flags, uint32_t *nmask, void *stream) {
+  LBDEM_REQUIRE(lat && flags && nmask, "lbdem_lbm_prepare_flags: null argument");
+  LBDEM_REQUIRE(lat->nx > 0 && lat->ny > 0 && lat->nz > 0, "lbdem_lbm_prepare_flags: bad extents");
+  const int64_t n = (int64_t)lat->nx * lat->ny * lat->nz;
+  LBDEM_REQUIRE(n < (int64_t(1) << 31), "lbdem_lbm_prepare_flags: too many cells");
+  KParams p{};
+  p.nx = lat->nx;
+  p.ny = lat->ny;
+  p.nz = lat->nz;
+  p.ncell = (int)n;
+  hipLaunchKernelGGL(k_prepare_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), p, flags,
+                     nmask);
+  LBDEM_CHECK_LAUNCH("lbdem_lbm_prepare_flags");
+  return LBDEM_OK;
+}
 
 const char *lbdem_last_error(void) { return g_last_error.c_str(); }
 
@@ -145,38 +185,21 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
   LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_VANS ||
                     (a->volfrac && a->hforce && a->volfrac_prev && a->hforce_prev && a->nu_lu > 0),
                 "lbdem_lbm_step: VANS needs volfrac, hforce, volfrac_prev, hforce_prev, nu_lu > 0");
-  // row-tiled variants: whole rows per workgroup, LDS staging of the z shift (<= 64 KiB).
-  // kernel 2: one cell per thread (two for 16-bit storage); kernel 3: two cells per thread.
-  // wave-shift variants: 256-thread workgroups over row segments, z shift by lane shuffles.
-  // kernels 4, 5, 6, 7: 1, 2, 4, 8 cells per thread (at most 16 bytes per access).
-  const bool half = lat->storage == LBDEM_STORE_F16 || lat->storage == LBDEM_STORE_BF16;
+  // row-tiled variants: whole rows per workgroup, LDS staging of the z shift (<= 64 KiB);
+  // kernel 2 / 3 / 4: one / two / four cells per thread.  Automatic choice (kernel 0), the
+  // fastest measured on MI355X: one cell per thread for FP32, two for FP64 and 16-bit storage.
   int kern = lat->kernel;
-  LBDEM_REQUIRE(kern >= 0 && kern <= 7, "lbdem_lbm_step: bad kernel variant %d", kern);
-  if (kern == 0) kern = 2;
-  if (kern >= 4) {  // wave-shift variants: any row length, nz % V == 0
-    const int Vw = 1 << (kern - 4);
-    LBDEM_REQUIRE(lat->nz % Vw == 0, "lbdem_lbm_step: %d cells per thread need nz %% %d == 0", Vw, Vw);
-    const KernelFn(*tab)[4][2] = kern == 4 ? t->wave1 : kern == 5 ? t->wave2 : kern == 6 ? t->wave4 : t->wave8;
-    KernelFn kw = tab[coll][lat->coupling][p.t_odd];
-    LBDEM_REQUIRE(kw, "lbdem_lbm_step: no wave-shift kernel with %d cells per thread for this storage", Vw);
-    const int bpr = (lat->nz + 256 * Vw - 1) / (256 * Vw);
-    void *args[] = {&p};
-    hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(kw), dim3((unsigned)(lat->nx * lat->ny * bpr)),
-                                   dim3(256), args, 0, as_stream(stream));
-    if (e != hipSuccess) {
-      set_error("lbdem_lbm_step: launch failed: %s", hipGetErrorString(e));
-      return LBDEM_E_HIP;
-    }
-    return LBDEM_OK;
-  }
-  const int V = (kern == 3 || half) ? 2 : 1;
+  LBDEM_REQUIRE(kern >= 0 && kern <= 4, "lbdem_lbm_step: bad kernel variant %d", kern);
+  if (kern == 0) kern = lat->storage == LBDEM_STORE_F32 ? 2 : 3;
+  const int V = kern == 4 ? 4 : kern == 3 ? 2 : 1;
   const int tsize = lat->storage == LBDEM_STORE_F64 ? 8 : 4;
   const int block = ((lat->nz / V + 63) / 64) * 64;
   const size_t lds = p.t_odd ? (size_t)10 * block * V * (tsize + 1) : 0;
   const bool pencil_ok = (lat->nz % V == 0) && block <= 1024 && lds <= 65536;
-  LBDEM_REQUIRE(lat->kernel < 2 || pencil_ok, "lbdem_lbm_step: row-tiled kernel needs nz %% %d == 0 and a row "
-                "of at most %d cells", V, 1024 * V);
-  KernelFn kp = (V == 2) ? t->pencil2[coll][lat->coupling][p.t_odd] : t->pencil[coll][lat->coupling][p.t_odd];
+  KernelFn kp = V == 4 ? t->pencil4[coll][lat->coupling][p.t_odd]
+                : V == 2 ? t->pencil2[coll][lat->coupling][p.t_odd] : t->pencil[coll][lat->coupling][p.t_odd];
+  LBDEM_REQUIRE(lat->kernel < 2 || (pencil_ok && kp), "lbdem_lbm_step: row-tiled kernel %d needs nz %% %d == 0, a "
+                "row of at most %d cells and (4 cells per thread) 32-bit or 16-bit storage", kern, V, 1024 * V);
   if (kern >= 2 && pencil_ok && kp) {
     void *args[] = {&p};
     hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(kp), dim3((unsigned)(lat->nx * lat->ny)),

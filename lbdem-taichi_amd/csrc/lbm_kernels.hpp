@@ -4,9 +4,11 @@
 // cell's own 19 slots (writing to the opposite slot); odd steps gather from the neighbours'
 // opposite slots and push to the neighbours' slots.  Each slot is touched by exactly one
 // thread per step, so the update is in place and race-free: 19 loads + 19 stores per cell.
-// Cells flagged NEAR (a wall neighbour or the domain edge) take a slow path that reads the
-// neighbour flags, bounces populations back at walls (halfway bounce-back, with the Ladd
-// momentum term at moving walls) and wraps periodically at the domain edge.
+// Cells flagged NEAR (a wall neighbour) and cells on the domain edge take a slow path that
+// bounces populations back at walls (halfway bounce-back, with the Ladd momentum term at moving
+// walls; the wall directions come from the cell's neighbour mask) and wraps periodically at the
+// domain edge.  The row-tiled kernels wrap by construction and take the slow path only for NEAR
+// cells.
 #pragma once
 #include <type_traits>
 #include <utility>
@@ -48,6 +50,11 @@ __device__ __forceinline__ T cdot(T x, T y, T z) {
 
 constexpr int kMaskSolid = LBDEM_FLAG_WALL | LBDEM_FLAG_MOVING;
 
+// all 18 neighbours lie inside the lattice without periodic wrap
+__device__ __forceinline__ bool interior(const KParams &p, int i, int j, int k) {
+  return i > 0 && i < p.nx - 1 && j > 0 && j < p.ny - 1 && k > 0 && k < p.nz - 1;
+}
+
 // Post-stream (shifted) populations of cell c after t steps, t parity `odd`.
 template <typename S>
 __device__ __forceinline__ void gather(const KParams &p, const S *__restrict__ F, int c, uint8_t fl,
@@ -58,24 +65,27 @@ __device__ __forceinline__ void gather(const KParams &p, const S *__restrict__ F
       constexpr int q = qc;
       f[q] = Store<S>::ld(F + q * N + c);
     });
-  } else if (!(fl & LBDEM_FLAG_NEAR)) {
-    const int sx = p.ny * p.nz, sy = p.nz;
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      const int off = kCX[q] * sx + kCY[q] * sy + kCZ[q];
-      f[q] = Store<S>::ld(F + kOPP[q] * N + (c - off));
-    });
   } else {
     int i, j, k;
     cell_ijk(p, c, i, j, k);
+    if (!(fl & LBDEM_FLAG_NEAR) && interior(p, i, j, k)) {
+      const int sx = p.ny * p.nz, sy = p.nz;
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        const int off = kCX[q] * sx + kCY[q] * sy + kCZ[q];
+        f[q] = Store<S>::ld(F + kOPP[q] * N + (c - off));
+      });
+      return;
+    }
+    const uint32_t nm = (fl & LBDEM_FLAG_NEAR) ? p.nmask[c] : 0u;
     static_for<0, Q>([&](auto qc) {
       constexpr int q = qc;
       if constexpr (q == 0) {
         f[0] = Store<S>::ld(F + c);
+      } else if ((nm >> kOPP[q]) & 1u) {
+        f[q] = Store<S>::ld(F + q * N + c);  // x - c_q is a wall: bounced back last step
       } else {
-        const int n = nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
-        if (p.flags[n] & kMaskSolid) f[q] = Store<S>::ld(F + q * N + c);  // bounced back last step
-        else f[q] = Store<S>::ld(F + kOPP[q] * N + n);
+        f[q] = Store<S>::ld(F + kOPP[q] * N + nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]));
       }
     });
   }
@@ -248,21 +258,42 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
   const T F3x = T(3) * rho * T(p.fx), F3y = T(3) * rho * T(p.fy), F3z = T(3) * rho * T(p.fz);
   const T vans_pref = T(1) - T(0.5) * omega;
 
-  if (!solid) {
+  if (!solid && COUP == LBDEM_COUPLE_VANS) {
     static_for<0, Q>([&](auto qc) {
       constexpr int q = qc;
       const T w = T(kW[q]);
       const T cu = cdot<q>(ux, uy, uz);
       const T feq = w * (dr + rf * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq));
       T nf = f[q] - omega * (f[q] - feq);
-      if constexpr (COUP == LBDEM_COUPLE_VANS) {
-        // F_q = w (1 - omega/2) (3c - 3u + 9 (c.u) c) . F  (src/unlbdem/vanslattice.py:278-318)
-        const T cF = cdot<q>(Fx, Fy, Fz);
-        nf += w * vans_pref * (T(3) * cF - T(3) * (ux * Fx + uy * Fy + uz * Fz) + T(9) * cu * cF);
-      } else {
-        if (forced) nf += w * cdot<q>(F3x, F3y, F3z);
-      }
+      // F_q = w (1 - omega/2) (3c - 3u + 9 (c.u) c) . F  (src/unlbdem/vanslattice.py:278-318)
+      const T cF = cdot<q>(Fx, Fy, Fz);
+      nf += w * vans_pref * (T(3) * cF - T(3) * (ux * Fx + uy * Fy + uz * Fz) + T(9) * cu * cF);
       f[q] = nf;
+    });
+  } else if (!solid) {
+    // BGK relaxation f <- (1 - omega) f + omega feq (+ w c.F3), by opposite pairs:
+    // omega feq_q = S_q + A_q with the even part S_q = omega w (A + 4.5 rf (c.u)^2) and the odd
+    // part A_q = omega w 3 rf (c.u), so the pair (q, qbar) shares S_q and A_qbar = -A_q.
+    const T omc = T(1) - omega;
+    const T oA = omega * (dr - T(1.5) * rf * usq);
+    const T k45 = omega * T(4.5) * rf, k3 = omega * T(3) * rf;
+    // per weight class (axis 1/18, diagonal 1/36): w oA, w k45, w k3
+    const T a_oA = T(kW[1]) * oA, a_k45 = T(kW[1]) * k45, a_k3 = T(kW[1]) * k3;
+    const T d_oA = T(kW[7]) * oA, d_k45 = T(kW[7]) * k45, d_k3 = T(kW[7]) * k3;
+    f[0] = omc * f[0] + T(kW[0]) * oA;
+    static_for<1, Q>([&](auto qc) {
+      constexpr int q = qc;
+      if constexpr (kOPP[q] > q) {
+        constexpr int r = kOPP[q];
+        constexpr bool axis = kW[q] == kW[1];
+        static_assert(axis || kW[q] == kW[7], "D3Q19 has two non-rest weight classes");
+        const T cu = cdot<q>(ux, uy, uz);
+        const T sym = (axis ? a_k45 : d_k45) * cu * cu + (axis ? a_oA : d_oA);
+        T anti = (axis ? a_k3 : d_k3) * cu;
+        if (forced) anti += T(kW[q]) * cdot<q>(F3x, F3y, F3z);
+        f[q] = omc * f[q] + (sym + anti);
+        f[r] = omc * f[r] + (sym - anti);
+      }
     });
   } else {
     // weighted collision fpc = f + B Omega_s + (1-B) Omega_f with
@@ -334,13 +365,13 @@ __global__ __launch_bounds__(256) void k_step(const KParams p) {
   if (!(fl & LBDEM_FLAG_ACTIVE)) return;
   S *__restrict__ F = reinterpret_cast<S *>(p.f);
   const size_t N = (size_t)p.ncell;
-  const bool near = fl & LBDEM_FLAG_NEAR;
+  int i, j, k;
+  cell_ijk(p, c, i, j, k);
+  const bool near = (fl & LBDEM_FLAG_NEAR) || !interior(p, i, j, k);  // bounce-back or periodic wrap
 
   T f[Q];
   gather<S>(p, F, c, fl, ODD, f);
 
-  int i = 0, j = 0, k = 0;
-  if (COUP == LBDEM_COUPLE_PSC || near) cell_ijk(p, c, i, j, k);
   const T rho = collide<COLL, COUP, T>(p, c, i, j, k, f);
 
   if (!near) {
@@ -358,16 +389,16 @@ __global__ __launch_bounds__(256) void k_step(const KParams p) {
       });
     }
   } else {
+    const uint32_t nm = (fl & LBDEM_FLAG_NEAR) ? p.nmask[c] : 0u;
     static_for<0, Q>([&](auto qc) {
       constexpr int q = qc;
       if constexpr (q == 0) {
         Store<S>::st(F + c, f[0]);
       } else {
         const int m = nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q]);
-        const uint8_t fm = p.flags[m];
-        if (fm & kMaskSolid) {
+        if ((nm >> q) & 1u) {
           T v = f[q];
-          if (fm & LBDEM_FLAG_MOVING) v -= ladd_term<q, T>(p, m, rho);
+          if ((nm & kNbrMoving) && (p.flags[m] & LBDEM_FLAG_MOVING)) v -= ladd_term<q, T>(p, m, rho);
           Store<S>::st(F + kOPP[q] * N + c, v);
         } else if constexpr (!ODD) {
           Store<S>::st(F + kOPP[q] * N + c, f[q]);
@@ -590,6 +621,7 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
   const int row = blockIdx.x;
   const int i = row / p.ny, j = row - (row / p.ny) * p.ny;
   const int k0 = threadIdx.x * V;
+  const unsigned uk0 = threadIdx.x * V;  // per-lane part of every row address (uniform row base)
   const bool lane_in = k0 < nz;
   S *__restrict__ F = reinterpret_cast<S *>(p.f);
   const size_t N = (size_t)p.ncell;
@@ -602,6 +634,9 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
     fl[v] = lane_in ? p.flags[base + k0 + v] : (uint8_t)0;
     act[v] = fl[v] & LBDEM_FLAG_ACTIVE;
   }
+  uint32_t nmv[V];  // wall-neighbour bits (NEAR cells only)
+#pragma unroll
+  for (int v = 0; v < V; ++v) nmv[v] = (fl[v] & LBDEM_FLAG_NEAR) ? p.nmask[base + k0 + v] : 0u;
   T f[V][Q];
 
   // ---------------------------------------------------------------- gather
@@ -610,7 +645,7 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
       static_for<0, Q>([&](auto qc) {
         constexpr int q = qc;
         T t[V];
-        Vec<S, V>::ld(F + q * N + base + k0, t);
+        Vec<S, V>::ld(F + (q * N + (size_t)base) + uk0, t);
 #pragma unroll
         for (int v = 0; v < V; ++v) f[v][q] = t[v];
       });
@@ -622,7 +657,7 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
       const int srow = wrapi(i - cx, p.nx) * p.ny + wrapi(j - cy, p.ny);
       if (lane_in) {
         T t[V];
-        Vec<S, V>::ld(F + kOPP[q] * N + (size_t)srow * nz + k0, t);
+        Vec<S, V>::ld(F + (kOPP[q] * N + (size_t)srow * nz) + uk0, t);
         if constexpr (cz == 0) {
 #pragma unroll
           for (int v = 0; v < V; ++v) f[v][q] = t[v];
@@ -650,10 +685,10 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
       for (int v = 0; v < V; ++v) {
         if (fl[v] & LBDEM_FLAG_NEAR) {  // populations bounced back at a wall last step
           const int k = k0 + v;
+          const uint32_t nm = nmv[v];
           static_for<1, Q>([&](auto qc) {
             constexpr int q = qc;
-            const int n = nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
-            if (p.flags[n] & kMaskSolid) f[v][q] = Store<S>::ld(F + q * N + base + k);
+            if ((nm >> kOPP[q]) & 1u) f[v][q] = Store<S>::ld(F + q * N + base + k);
           });
         }
       }
@@ -677,13 +712,13 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           t[v] = f[v][q];
-          if (q != 0 && (fl[v] & LBDEM_FLAG_NEAR)) {
+          if (q != 0 && ((nmv[v] >> q) & 1u) && (nmv[v] & kNbrMoving)) {
             const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
             if (p.flags[m] & LBDEM_FLAG_MOVING) t[v] -= ladd_term<q, T>(p, m, rho[v]);
           }
         }
         // inactive cells' slots are never read, so they may receive garbage here
-        Vec<S, V>::st(F + kOPP[q] * N + base + k0, t);
+        Vec<S, V>::st(F + (kOPP[q] * N + (size_t)base) + uk0, t);
       });
     }
   } else {
@@ -697,19 +732,18 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           ok[v] = act[v];
-          if (q != 0 && act[v] && (fl[v] & LBDEM_FLAG_NEAR)) {
-            const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
-            const uint8_t fm = p.flags[m];
-            if (fm & kMaskSolid) {  // bounce back into the own opposite slot
-              T val = f[v][q];
-              if (fm & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
-              Store<S>::st(F + kOPP[q] * N + base + k0 + v, val);
-              ok[v] = false;
+          if (q != 0 && act[v] && ((nmv[v] >> q) & 1u)) {  // bounce back into the own opposite slot
+            T val = f[v][q];
+            if (nmv[v] & kNbrMoving) {
+              const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
+              if (p.flags[m] & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
             }
+            Store<S>::st(F + kOPP[q] * N + base + k0 + v, val);
+            ok[v] = false;
           }
         }
         if constexpr (cz == 0) {
-          S *dst = F + q * N + (size_t)trow * nz + k0;
+          S *dst = F + (q * N + (size_t)trow * nz) + uk0;
           bool all = true;
 #pragma unroll
           for (int v = 0; v < V; ++v) all = all && ok[v];
@@ -739,7 +773,7 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
         constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
         if constexpr (cz != 0) {
           const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
-          S *dst = F + q * N + (size_t)trow * nz + k0;
+          S *dst = F + (q * N + (size_t)trow * nz) + uk0;
           T t[V];
           bool ok[V], all = true;
 #pragma unroll
@@ -763,214 +797,11 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
   }
 }
 
-}  // namespace lbdem
-
-// ---------------------------------------------------------------------------------------
-// Wave-shift variant: 256 threads cover 256 V consecutive cells of one (i, j) row (a row
-// longer than that takes several workgroups).  All row accesses are aligned; the +-1 shift
-// along k of the odd step moves values between neighbouring lanes of a wavefront with a
-// shuffle, and only the element that crosses a wavefront boundary (or wraps around the row)
-// is loaded or stored on its own.  No LDS staging and no barriers.
-// ---------------------------------------------------------------------------------------
-namespace lbdem {
-
-__device__ __forceinline__ int wrapk(int k, int nz) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); }
-
-template <typename T>
-__device__ __forceinline__ T lane_shfl(T v, int src) {
-  if constexpr (sizeof(T) == 8) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, src, 64);
-    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), src, 64);
-    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
-  } else {
-    return __builtin_bit_cast(T, __shfl(__builtin_bit_cast(int, v), src, 64));
-  }
-}
-
+// row-tiled kernel with V cells per thread, or nullptr where one access would exceed 16 bytes
 template <typename S, int COLL, int COUP, bool ODD, int V>
-__global__ __launch_bounds__(256) void k_step_wave(const KParams p) {
-  using T = typename Store<S>::T;
-  const int nz = p.nz;
-  const int seg_cells = 256 * V;
-  const int bpr = (nz + seg_cells - 1) / seg_cells;
-  const int row = blockIdx.x / bpr;
-  const int seg = blockIdx.x - row * bpr;
-  const int i = row / p.ny, j = row - (row / p.ny) * p.ny;
-  const int lane = threadIdx.x & 63;
-  const int k0 = seg * seg_cells + threadIdx.x * V;            // first of this lane's V cells
-  const int wk0 = seg * seg_cells + (threadIdx.x & ~63) * V;   // first cell of the wavefront
-  const int wk1 = min(wk0 + 64 * V, nz) - 1;                   // last valid cell of the wavefront
-  const bool lane_in = k0 < nz;                                // nz % V == 0: all V cells valid
-  const int prv = (lane + 63) & 63, nxt = (lane + 1) & 63;
-  S *__restrict__ F = reinterpret_cast<S *>(p.f);
-  const size_t N = (size_t)p.ncell;
-  const int base = row * nz;
-
-  uint8_t fl[V];
-  bool act[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) {
-    fl[v] = lane_in ? p.flags[base + k0 + v] : (uint8_t)0;
-    act[v] = fl[v] & LBDEM_FLAG_ACTIVE;
-  }
-  T f[V][Q];
-
-  // ---------------------------------------------------------------- gather
-  if constexpr (!ODD) {
-    if (lane_in) {
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = qc;
-        T t[V];
-        Vec<S, V>::ld(F + q * N + base + k0, t);
-#pragma unroll
-        for (int v = 0; v < V; ++v) f[v][q] = t[v];
-      });
-    }
-  } else {
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      const S *src = F + kOPP[q] * N + (size_t)(wrapi(i - cx, p.nx) * p.ny + wrapi(j - cy, p.ny)) * nz;
-      T a[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) a[v] = T(0);
-      if (lane_in) Vec<S, V>::ld(src + k0, a);
-      if constexpr (cz == 0) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) f[v][q] = a[v];
-      } else if constexpr (cz == 1) {
-        // cells k0 - 1 .. k0 + V - 2: the previous lane's last element, then own 0 .. V - 2
-        const T s = lane_shfl(a[V - 1], prv);
-        f[0][q] = (k0 - 1 < wk0) ? (lane_in ? Store<S>::ld(src + wrapk(k0 - 1, nz)) : T(0)) : s;
-#pragma unroll
-        for (int v = 1; v < V; ++v) f[v][q] = a[v - 1];
-      } else {
-        // cells k0 + 1 .. k0 + V: own 1 .. V - 1, then the next lane's first element
-        const T s = lane_shfl(a[0], nxt);
-#pragma unroll
-        for (int v = 0; v < V - 1; ++v) f[v][q] = a[v + 1];
-        f[V - 1][q] = (k0 + V > wk1) ? (lane_in ? Store<S>::ld(src + wrapk(k0 + V, nz)) : T(0)) : s;
-      }
-    });
-    if (lane_in) {
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        if (fl[v] & LBDEM_FLAG_NEAR) {  // populations bounced back at a wall last step
-          const int k = k0 + v;
-          static_for<1, Q>([&](auto qc) {
-            constexpr int q = qc;
-            const int n = nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]);
-            if (p.flags[n] & kMaskSolid) f[v][q] = Store<S>::ld(F + q * N + base + k);
-          });
-        }
-      }
-    }
-  }
-
-  // ---------------------------------------------------------------- collide
-  T rho[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) {
-    rho[v] = T(0);
-    if (act[v]) rho[v] = collide<COLL, COUP, T>(p, base + k0 + v, i, j, k0 + v, f[v]);
-  }
-
-  // ---------------------------------------------------------------- scatter
-  if constexpr (!ODD) {
-    if (lane_in) {
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = qc;
-        T t[V];
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          t[v] = f[v][q];
-          if (q != 0 && (fl[v] & LBDEM_FLAG_NEAR)) {
-            const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
-            if (p.flags[m] & LBDEM_FLAG_MOVING) t[v] -= ladd_term<q, T>(p, m, rho[v]);
-          }
-        }
-        Vec<S, V>::st(F + kOPP[q] * N + base + k0, t);  // inactive cells' slots are never read
-      });
-    }
-  } else {
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      S *dst = F + q * N + (size_t)(wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny)) * nz;
-      bool ok[V];
-      T val[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        val[v] = f[v][q];
-        ok[v] = act[v];
-        if (q != 0 && act[v] && (fl[v] & LBDEM_FLAG_NEAR)) {
-          const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
-          const uint8_t fm = p.flags[m];
-          if (fm & kMaskSolid) {  // bounce back into the own opposite slot
-            T b = val[v];
-            if (fm & LBDEM_FLAG_MOVING) b -= ladd_term<q, T>(p, m, rho[v]);
-            Store<S>::st(F + kOPP[q] * N + base + k0 + v, b);
-            ok[v] = false;
-          }
-        }
-      }
-      // out[v]: value for dst[k0 + v], stored when oko[v]
-      T out[V];
-      bool oko[V];
-      if constexpr (cz == 0) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          out[v] = val[v];
-          oko[v] = ok[v];
-        }
-      } else if constexpr (cz == 1) {
-        // positions k0 .. k0 + V - 1 receive cells k0 - 1 .. k0 + V - 2
-        const T s = lane_shfl(val[V - 1], prv);
-        const int so = __shfl((int)ok[V - 1], prv, 64);
-        out[0] = s;
-        oko[0] = lane_in && (k0 - 1 >= wk0) && so;
-#pragma unroll
-        for (int v = 1; v < V; ++v) {
-          out[v] = val[v - 1];
-          oko[v] = ok[v - 1];
-        }
-        // the last cell's value leaves this wavefront (or wraps): stored by its own lane
-        if (lane_in && ok[V - 1] && k0 + V > wk1) Store<S>::st(dst + wrapk(k0 + V, nz), val[V - 1]);
-      } else {
-        // positions k0 .. k0 + V - 1 receive cells k0 + 1 .. k0 + V
-        const T s = lane_shfl(val[0], nxt);
-        const int so = __shfl((int)ok[0], nxt, 64);
-#pragma unroll
-        for (int v = 0; v < V - 1; ++v) {
-          out[v] = val[v + 1];
-          oko[v] = ok[v + 1];
-        }
-        out[V - 1] = s;
-        oko[V - 1] = lane_in && (k0 + V <= wk1) && so;
-        if (lane_in && ok[0] && k0 - 1 < wk0) Store<S>::st(dst + wrapk(k0 - 1, nz), val[0]);
-      }
-      if (lane_in) {
-        bool all = true;
-#pragma unroll
-        for (int v = 0; v < V; ++v) all = all && oko[v];
-        if (all) {
-          Vec<S, V>::st(dst + k0, out);
-        } else {
-#pragma unroll
-          for (int v = 0; v < V; ++v)
-            if (oko[v]) Store<S>::st(dst + k0 + v, out[v]);
-        }
-      }
-    });
-  }
-}
-
-// wave-shift kernel with V cells per thread, or nullptr where one access would exceed 16 bytes
-template <typename S, int COLL, int COUP, bool ODD, int V>
-constexpr void (*wave_fn())(const KParams) {
+constexpr void (*pencil_fn())(const KParams) {
   if constexpr (V * sizeof(S) <= 16) {
-    return k_step_wave<S, COLL, COUP, ODD, V>;
+    return k_step_pencil<S, COLL, COUP, ODD, V>;
   } else {
     return nullptr;
   }

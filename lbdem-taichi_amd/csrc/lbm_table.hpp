@@ -10,10 +10,7 @@ struct LbmKernels {
   KernelFn step[3][4][2];  // [collision][coupling][odd]; nullptr = unsupported combination
   KernelFn pencil[3][4][2];   // row-tiled variant of step, one cell per thread
   KernelFn pencil2[3][4][2];  // row-tiled variant, two cells per thread
-  KernelFn wave1[3][4][2];    // wave-shift variant, one cell per thread
-  KernelFn wave2[3][4][2];    // wave-shift variant, two cells per thread
-  KernelFn wave4[3][4][2];    // wave-shift variant, four cells per thread (<= 16 B per access)
-  KernelFn wave8[3][4][2];    // wave-shift variant, eight cells per thread (16-bit storage)
+  KernelFn pencil4[3][4][2];  // row-tiled variant, four cells per thread (not for FP64)
   KernelFn init[4];        // [coupling]
   KernelFn macros[4];      // [coupling]
   KernelFn read_f;
@@ -37,115 +34,67 @@ const LbmKernels &lbm_kernels_f64();
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false>;               \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 1>;               \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 2>;               \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 1>;               \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 2>;               \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 4>();               \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 8>();               \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][0] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, false, 4>();               \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true>;                \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 1>;                \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 2>;                \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 1>;                \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 2>;                \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 4>();                \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 8>();                \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_NONE][1] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_NONE, true, 4>();                \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false>; \
       k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 1>; \
       k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 2>; \
-      k.wave1[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 1>; \
-      k.wave2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 2>; \
-      k.wave4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 4>(); \
-      k.wave8[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 8>(); \
+      k.pencil4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][0] = pencil_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, false, 4>(); \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true>;  \
       k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 1>;  \
       k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 2>;  \
-      k.wave1[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 1>;  \
-      k.wave2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 2>;  \
-      k.wave4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 4>();  \
-      k.wave8[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 8>();  \
+      k.pencil4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_NONE][1] = pencil_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_NONE, true, 4>();  \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false>;   \
       k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 1>;   \
       k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 2>;   \
-      k.wave1[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 1>;   \
-      k.wave2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 2>;   \
-      k.wave4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 4>();   \
-      k.wave8[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 8>();   \
+      k.pencil4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][0] = pencil_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, false, 4>();   \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true>;    \
       k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 1>;    \
       k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 2>;    \
-      k.wave1[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 1>;    \
-      k.wave2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 2>;    \
-      k.wave4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 4>();    \
-      k.wave8[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 8>();    \
+      k.pencil4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_NONE][1] = pencil_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_NONE, true, 4>();    \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false>;                 \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 1>;                 \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 2>;                 \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 1>;                 \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 2>;                 \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 4>();                 \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 8>();                 \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][0] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, false, 4>();                 \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true>;                  \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 1>;                  \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 2>;                  \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 1>;                  \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 2>;                  \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 4>();                  \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 8>();                  \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_PSC][1] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_PSC, true, 4>();                  \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false>;             \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 1>;             \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 2>;             \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 1>;             \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 2>;             \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 4>();             \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 8>();             \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][0] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, false, 4>();             \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true>;              \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 1>;              \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 2>;              \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 1>;              \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 2>;              \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 4>();              \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 8>();              \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_EQIBM][1] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_EQIBM, true, 4>();              \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false>; \
       k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 1>; \
       k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 2>; \
-      k.wave1[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 1>; \
-      k.wave2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 2>; \
-      k.wave4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 4>(); \
-      k.wave8[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 8>(); \
+      k.pencil4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][0] = pencil_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, false, 4>(); \
       k.step[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true>;  \
       k.pencil[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 1>;  \
       k.pencil2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 2>;  \
-      k.wave1[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 1>;  \
-      k.wave2[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = k_step_wave<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 2>;  \
-      k.wave4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 4>();  \
-      k.wave8[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = wave_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 8>();  \
+      k.pencil4[LBDEM_COLL_SMAG_LAGGED][LBDEM_COUPLE_EQIBM][1] = pencil_fn<S, LBDEM_COLL_SMAG_LAGGED, LBDEM_COUPLE_EQIBM, true, 4>();  \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false>;   \
       k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 1>;   \
       k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 2>;   \
-      k.wave1[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 1>;   \
-      k.wave2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 2>;   \
-      k.wave4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 4>();   \
-      k.wave8[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 8>();   \
+      k.pencil4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][0] = pencil_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, false, 4>();   \
       k.step[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true>;    \
       k.pencil[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 1>;    \
       k.pencil2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step_pencil<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 2>;    \
-      k.wave1[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 1>;    \
-      k.wave2[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = k_step_wave<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 2>;    \
-      k.wave4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 4>();    \
-      k.wave8[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = wave_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 8>();    \
+      k.pencil4[LBDEM_COLL_SMAG_FIXED][LBDEM_COUPLE_EQIBM][1] = pencil_fn<S, LBDEM_COLL_SMAG_FIXED, LBDEM_COUPLE_EQIBM, true, 4>();    \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false>;               \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 1>;               \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 2>;               \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 1>;               \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 2>;               \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 4>();               \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 8>();               \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][0] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, false, 4>();               \
       k.step[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true>;                \
       k.pencil[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 1>;                \
       k.pencil2[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step_pencil<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 2>;                \
-      k.wave1[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 1>;                \
-      k.wave2[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = k_step_wave<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 2>;                \
-      k.wave4[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 4>();                \
-      k.wave8[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = wave_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 8>();                \
+      k.pencil4[LBDEM_COLL_BGK][LBDEM_COUPLE_VANS][1] = pencil_fn<S, LBDEM_COLL_BGK, LBDEM_COUPLE_VANS, true, 4>();                \
       k.init[LBDEM_COUPLE_NONE] = k_init<S, LBDEM_COUPLE_NONE>;                                 \
       k.init[LBDEM_COUPLE_PSC] = k_init<S, LBDEM_COUPLE_NONE>;                                  \
       k.init[LBDEM_COUPLE_EQIBM] = k_init<S, LBDEM_COUPLE_NONE>;                                \

@@ -131,17 +131,12 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
   }
 }
 
-__device__ __forceinline__ double block_sum256(double v, double *sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wv] = v;
-  __syncthreads();
-  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
-}
+constexpr int kForceThreads = 1024;
 
-// lattice2grains (psclattice3d.py:316-358): FP64, fixed reduction order (deterministic)
-__global__ __launch_bounds__(256) void k_psc_forces(const PscArgs a) {
+// lattice2grains (psclattice3d.py:316-358): one 1024-thread workgroup per grain, FP64, fixed
+// reduction order (per-thread sums, wavefront shuffles, then the 16 wave sums pairwise in LDS),
+// so the result is deterministic.
+__global__ __launch_bounds__(kForceThreads) void k_psc_forces(const PscArgs a) {
   const int g = blockIdx.x;
   double xc, yc, zc;
   int b[6];
@@ -152,7 +147,7 @@ __global__ __launch_bounds__(256) void k_psc_forces(const PscArgs a) {
   const uint32_t gid = (uint32_t)(kMaxGrain - g);
   if (ni > 0 && nj > 0 && nk > 0) {
     const int tot = ni * nj * nk;
-    for (int t = threadIdx.x; t < tot; t += 256) {
+    for (int t = threadIdx.x; t < tot; t += kForceThreads) {
       const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
       const int c = (i * a.ny + j) * a.nz + k;
       const uint32_t key = a.key[c];
@@ -168,18 +163,29 @@ __global__ __launch_bounds__(256) void k_psc_forces(const PscArgs a) {
       T[2] -= d0 * f1 - d1 * f0;
     }
   }
-  __shared__ double sh[4];
-  double out[6];
+  constexpr int kWaves = kForceThreads / 64;
+  __shared__ double sh[6][kWaves];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) out[m] = block_sum256(F[m], sh);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) out[3 + m] = block_sum256(T[m], sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      a.force_fluid[3 * g + m] = out[m];
-      a.moment_fluid[3 * g + m] = out[3 + m];
+  for (int m = 0; m < 3; ++m) {
+    const double fs = wave_sum(F[m]), ts = wave_sum(T[m]);
+    if (lane == 0) {
+      sh[m][wv] = fs;
+      sh[3 + m][wv] = ts;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v[kWaves];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) v[w] = sh[threadIdx.x][w];
+#pragma unroll
+    for (int h = kWaves / 2; h > 0; h /= 2)
+#pragma unroll
+      for (int w = 0; w < h; ++w) v[w] += v[w + h];
+    const int m = threadIdx.x;
+    if (m < 3) a.force_fluid[3 * g + m] = v[0];
+    else a.moment_fluid[3 * g + m - 3] = v[0];
   }
 }
 
@@ -283,7 +289,7 @@ int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbde
   if (g->n == 0) return LBDEM_OK;
   a.key = const_cast<uint32_t *>(key);
   a.cell_force = cell_force;
-  hipLaunchKernelGGL(k_psc_forces, dim3(g->n), dim3(256), 0, as_stream(stream), a);
+  hipLaunchKernelGGL(k_psc_forces, dim3(g->n), dim3(kForceThreads), 0, as_stream(stream), a);
   LBDEM_CHECK_LAUNCH("lbdem_psc_forces");
   return LBDEM_OK;
 }

@@ -27,8 +27,11 @@ class Lattice(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+ABI_VERSION = 2
+
+
 class StepArgs(ctypes.Structure):
-    _fields_ = [("f", _vp), ("flags", _vp), ("vel", _vp), ("lag", _vp),
+    _fields_ = [("f", _vp), ("flags", _vp), ("nmask", _vp), ("vel", _vp), ("lag", _vp),
                 ("psc_key", _vp), ("psc_grain", _vp), ("psc_force", _vp),
                 ("volfrac", _vp), ("velsolid", _vp), ("hforce", _vp),
                 ("volfrac_prev", _vp), ("hforce_prev", _vp),
@@ -58,13 +61,14 @@ def lib():
         "lbdem_lbm_write_f": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, ctypes.c_int32, _vp]),
         "lbdem_reduce_work_doubles": (ctypes.c_int64, [P(Lattice)]),
         "lbdem_lbm_reduce": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, _vp]),
+        "lbdem_lbm_prepare_flags": (ctypes.c_int, [P(Lattice), _vp, _vp, _vp]),
     }
     sig.update(_EXTRA_SIGNATURES)
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    if L.lbdem_abi_version() != 1:
+    if L.lbdem_abi_version() != ABI_VERSION:
         raise RuntimeError("liblbdem_hip.so ABI version mismatch")
     _lib = L
     return L

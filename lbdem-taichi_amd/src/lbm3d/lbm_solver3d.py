@@ -103,6 +103,7 @@ class BasicLattice3D:
 
         self._f = torch.zeros(19 * self.N, dtype=_hip.STORE_DTYPE[self.storage], device=dev)
         self._flags = torch.zeros(self.N, dtype=torch.uint8, device=dev)
+        self._nmask = torch.zeros(self.N, dtype=torch.int32, device=dev)  # wall-neighbour bits
         self._lag = None
         if self._collision() == _hip.COLL_SMAG_LAGGED:
             lag_dtype = torch.float64 if self.storage == "f64" else torch.float32
@@ -149,6 +150,7 @@ class BasicLattice3D:
         a = self._args
         a.f = self._f.data_ptr()
         a.flags = self._flags.data_ptr()
+        a.nmask = self._nmask.data_ptr()
         a.vel = self.vel.dev.data_ptr()
         a.lag = self._lag.data_ptr() if self._lag is not None else None
         a.t = self.t
@@ -172,16 +174,15 @@ class BasicLattice3D:
         wall = kind == CellType.OBSTACLE
         moving = kind == CellType.VEL_LADD
         fluid = kind == CellType.FLUID
-        near = np.zeros_like(active)
-        for cx, cy, cz in self.c[1:].astype(int):
-            near |= ~np.roll(active, shift=(-cx, -cy, -cz), axis=(0, 1, 2))
-        near[0, :, :] = near[-1, :, :] = True
-        near[:, 0, :] = near[:, -1, :] = True
-        near[:, :, 0] = near[:, :, -1] = True
-        near &= active
-        flags = (active * _hip.FLAG_ACTIVE | near * _hip.FLAG_NEAR | wall * _hip.FLAG_WALL |
-                 moving * _hip.FLAG_MOVING | fluid * _hip.FLAG_FLUID).astype(np.uint8)
+        flags = (active * _hip.FLAG_ACTIVE | wall * _hip.FLAG_WALL | moving * _hip.FLAG_MOVING |
+                 fluid * _hip.FLAG_FLUID).astype(np.uint8)
         self._flags.copy_(torch.from_numpy(flags.reshape(-1)))
+        dims = _hip.Lattice()
+        dims.nx, dims.ny, dims.nz = self.Nx, self.Ny, self.Nz
+        # NEAR bits and wall-neighbour masks are derived on the device
+        _hip.check(_hip.lib().lbdem_lbm_prepare_flags(ctypes.byref(dims), _hip.ptr(self._flags),
+                                                       _hip.ptr(self._nmask), _hip.stream()),
+                   "lbdem_lbm_prepare_flags")
         self._ct_version = self.CT.version
         self._active_np = active
 

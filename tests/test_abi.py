@@ -28,7 +28,8 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, f"not exported: {missing}"
     L.lbdem_abi_version.restype = ctypes.c_int
-    assert L.lbdem_abi_version() == 1
+    import src._hip as H
+    assert L.lbdem_abi_version() == H.ABI_VERSION
 
 
 def test_python_binding_declares_every_header_function():
@@ -43,7 +44,8 @@ def test_python_binding_declares_every_header_function():
             pass
     known = set(H._EXTRA_SIGNATURES) | {
         "lbdem_abi_version", "lbdem_last_error", "lbdem_lbm_init", "lbdem_lbm_step", "lbdem_lbm_macros",
-        "lbdem_lbm_read_f", "lbdem_lbm_write_f", "lbdem_reduce_work_doubles", "lbdem_lbm_reduce"}
+        "lbdem_lbm_read_f", "lbdem_lbm_write_f", "lbdem_reduce_work_doubles", "lbdem_lbm_reduce",
+        "lbdem_lbm_prepare_flags"}
     missing = [n for n in declared_functions() if n not in known]
     assert not missing, f"no ctypes prototype for {missing}"
 
@@ -58,3 +60,19 @@ def test_errors_are_reported_without_a_gpu():
     rc = L.lbdem_lbm_step(ctypes.byref(lat), ctypes.byref(args), None)
     assert rc == -1
     assert b"bad extents" in L.lbdem_last_error()
+
+
+def test_header_constants_match_python_binding():
+    """Flag bits, storage / collision / coupling codes: header and ctypes binding agree."""
+    import src._hip as H
+    text = open(HEADER).read()
+    consts = {m.group(1): int(m.group(2), 0) for m in re.finditer(r"#define\s+(LBDEM_[A-Z0-9_]+)\s+(0x[0-9a-fA-F]+|\d+)",
+                                                                      text)}
+    for name in ("ACTIVE", "NEAR", "WALL", "MOVING", "FLUID"):
+        assert consts[f"LBDEM_FLAG_{name}"] == getattr(H, f"FLAG_{name}"), name
+    assert len({consts[f"LBDEM_FLAG_{n}"] for n in ("ACTIVE", "NEAR", "WALL", "MOVING", "FLUID")}) == 5
+    for key, code in H.STORE.items():
+        assert consts[f"LBDEM_STORE_{key.upper()}"] == code, key
+    for name in ("COLL_BGK", "COLL_SMAG_LAGGED", "COLL_SMAG_FIXED", "COUPLE_NONE", "COUPLE_PSC", "COUPLE_EQIBM",
+                 "COUPLE_VANS"):
+        assert consts[f"LBDEM_{name}"] == getattr(H, name), name

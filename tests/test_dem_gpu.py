@@ -52,11 +52,11 @@ def test_broadphase_empty_and_single():
     assert got.shape == (0, 2)
 
 
-def _run_pair(model, path, n=120, steps=400, periodic=(False, False, False), seed=1):
+def _run_pair(model, path, n=120, steps=400, periodic=(False, False, False), seed=1, r=0.045):
     rng = np.random.default_rng(seed)
     box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     cfg = make_config(model=model, box=box, dt=5e-5, periodic=periodic, kn=2e3, kt=2e3, zeta=0.3)
-    rad, mass, pos, vel = random_packing(n, 0.045, box, rng, v=0.5, polydisperse=0.3)
+    rad, mass, pos, vel = random_packing(n, r, box, rng, v=0.5, polydisperse=0.3)
     gpu = gpu_solver(cfg, rad, mass, pos, vel, path=path)
     orc = oracle_solver(cfg, rad, mass, pos, vel)
     for _ in range(steps // 50):
@@ -92,6 +92,15 @@ def test_dem_trajectories_match_oracle(model, path):
 def test_dem_periodic_axis():
     gpu, orc = _run_pair("hertz", 0, periodic=(True, False, False), seed=4)
     assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+
+
+def test_dem_small_path_state_in_global_memory():
+    """600 particles: the single-workgroup path with the state in global memory (above the
+    LDS-resident size) against the oracle."""
+    gpu, orc = _run_pair("hertz", 1, n=600, seed=11, r=0.03)
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+    assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
+    assert gpu.status()["slot_overflow"] == 0
 
 
 def test_dem_small_and_general_paths_agree():
