@@ -452,7 +452,8 @@ def main():
                    "grains": NGRAIN, "seed": args.seed, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "fused PSC collide-stream (k_step_pencil<float, BGK, PSC>)",
+                     "kernel": "fused PSC collide-stream, row-tiled, one cell per thread "
+                               "(k_step_pencil<float, BGK, PSC, V=1>)",
                      "kernel_ms": round(k_ms, 4), "bytes_per_launch": alg_bytes},
         "wall_s": round(wall, 3),
         "extras": extras,
