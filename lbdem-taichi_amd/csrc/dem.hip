@@ -661,13 +661,16 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   __shared__ int nnew;
   const int n = P.n;
   lbdem_dem_state S = G;
-  // 3-vectors and scalars copied to LDS: pos, vel, acc, force, omega, omega_dot, moment, quat(4),
-  // mass, radius, cfn
+  // copied to LDS: pos, vel, acc, force, omega, omega_dot, moment, quat(4), mass, radius, cfn
+  // (read and written), inertia, force_fluid, moment_fluid, freeze, fixvel (read only)
   constexpr int kRes = RESIDENT ? kResidentN : 1;
   __shared__ double r_vec[7][3 * kRes];
   __shared__ double r_quat[4 * kRes];
   __shared__ double r_mass[kRes], r_rad[kRes];
   __shared__ int32_t r_cfn[kRes];
+  __shared__ double r_inertia[9 * kRes];
+  __shared__ double r_fluid[2][3 * kRes];
+  __shared__ int32_t r_freeze[kRes], r_fixvel[kRes];
   if constexpr (RESIDENT) {
     double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
     for (int t = threadIdx.x; t < 3 * n; t += NT) {
@@ -675,10 +678,17 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
       for (int a = 0; a < 7; ++a) r_vec[a][t] = gv[a][t];
     }
     for (int t = threadIdx.x; t < 4 * n; t += NT) r_quat[t] = G.quat[t];
+    for (int t = threadIdx.x; t < 9 * n; t += NT) r_inertia[t] = G.inertia[t];
+    for (int t = threadIdx.x; t < 3 * n; t += NT) {
+      r_fluid[0][t] = G.force_fluid[t];
+      r_fluid[1][t] = G.moment_fluid[t];
+    }
     for (int t = threadIdx.x; t < n; t += NT) {
       r_mass[t] = G.mass[t];
       r_rad[t] = G.radius[t];
       r_cfn[t] = G.cfn[t];
+      r_freeze[t] = G.freeze ? G.freeze[t] : 0;
+      r_fixvel[t] = G.fixvel ? G.fixvel[t] : 0;
     }
     S.pos = r_vec[0];
     S.vel = r_vec[1];
@@ -691,6 +701,11 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     S.mass = r_mass;
     S.radius = r_rad;
     S.cfn = r_cfn;
+    S.inertia = r_inertia;
+    S.force_fluid = r_fluid[0];
+    S.moment_fluid = r_fluid[1];
+    if (G.freeze) S.freeze = r_freeze;
+    if (G.fixvel) S.fixvel = r_fixvel;
   }
   for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;
   __syncthreads();

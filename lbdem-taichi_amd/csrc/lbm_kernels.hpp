@@ -91,6 +91,26 @@ __device__ __forceinline__ void gather(const KParams &p, const S *__restrict__ F
   }
 }
 
+// 1/x in the collision: the hardware reciprocal (1 ulp) for FP32, a division for FP64
+template <typename T>
+__device__ __forceinline__ T recip(T x) {
+  if constexpr (std::is_same_v<T, float>) {
+    return __builtin_amdgcn_rcpf(x);
+  } else {
+    return T(1) / x;
+  }
+}
+
+// square root in the collision: the hardware instruction (1 ulp) for FP32, IEEE for FP64
+template <typename T>
+__device__ __forceinline__ T root(T x) {
+  if constexpr (std::is_same_v<T, float>) {
+    return __builtin_amdgcn_sqrtf(x);
+  } else {
+    return sqrt(x);
+  }
+}
+
 // Tenneti et al. (2011) drag weight of the particle-equivalence IBM, in lattice units.
 // Follows EqIMBlattice3D.compute_weight / weight_coefficient (src/unlbdem/eqlattice.py:200-238):
 // R = (3 eps / 4 pi)^(1/3), d = 2R, Re = (1-eps) d |u_s - u| / nu, W = 3 pi d nu (1-eps) C_d.
@@ -126,7 +146,7 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
     acc<kCZ[q]>(jz, f[q]);
   });
   const T rho = T(p.rho0) + dr;
-  const T inv = T(1) / rho;
+  const T inv = recip(rho);
   T ux = jx * inv, uy = jy * inv, uz = jz * inv;
 
   T fvf = T(1);  // fluid volume fraction in the VANS equilibrium
@@ -196,12 +216,12 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
     const T nxy = pxy - er * eux * euy;
     const T nxz = pxz - er * eux * euz;
     const T nyz = pyz - er * euy * euz;
-    const T Pi = sqrt(T(2) * (nxx * nxx + nyy * nyy + nzz * nzz + T(2) * (nxy * nxy + nxz * nxz + nyz * nyz)));
+    const T Pi = root(T(2) * (nxx * nxx + nyy * nyy + nzz * nzz + T(2) * (nxy * nxy + nxz * nxz + nyz * nyz)));
     if constexpr (COLL == LBDEM_COLL_SMAG_LAGGED) {
       // rate = omega_old Pi / (2 cs^2 rho); tau = tau0 + Cs^2 rate / cs^2
       // (computeOmega, src/lbm3d/lbm_solver3d.py:500-513)
       const T rate = omega_old * Pi * T(1.5) * inv;
-      omega = T(1) / (T(p.tau0) + T(3) * T(p.cs2_smag) * rate);
+      omega = recip(T(p.tau0) + T(3) * T(p.cs2_smag) * rate);
       const size_t N = (size_t)p.ncell;
       T *lag = reinterpret_cast<T *>(p.lag);
       lag[c] = rho;
@@ -212,7 +232,7 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
     } else {
       // tau = tau0 + 9/2 Cs^2 Pi / (rho tau)  =>  tau = (tau0 + sqrt(tau0^2 + 18 Cs^2 Pi/rho)) / 2
       const T tau0 = T(p.tau0);
-      omega = T(2) / (tau0 + sqrt(tau0 * tau0 + T(18) * T(p.cs2_smag) * Pi * inv));
+      omega = T(2) * recip(tau0 + root(tau0 * tau0 + T(18) * T(p.cs2_smag) * Pi * inv));
     }
   }
 
@@ -635,8 +655,12 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
     act[v] = fl[v] & LBDEM_FLAG_ACTIVE;
   }
   uint32_t nmv[V];  // wall-neighbour bits (NEAR cells only)
+  uint32_t nm_any = 0u;  // union over this thread's cells: the wall paths below branch on it
 #pragma unroll
-  for (int v = 0; v < V; ++v) nmv[v] = (fl[v] & LBDEM_FLAG_NEAR) ? p.nmask[base + k0 + v] : 0u;
+  for (int v = 0; v < V; ++v) {
+    nmv[v] = (fl[v] & LBDEM_FLAG_NEAR) ? p.nmask[base + k0 + v] : 0u;
+    nm_any |= nmv[v];
+  }
   T f[V][Q];
 
   // ---------------------------------------------------------------- gather
@@ -710,11 +734,14 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
         constexpr int q = qc;
         T t[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          t[v] = f[v][q];
-          if (q != 0 && ((nmv[v] >> q) & 1u) && (nmv[v] & kNbrMoving)) {
-            const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
-            if (p.flags[m] & LBDEM_FLAG_MOVING) t[v] -= ladd_term<q, T>(p, m, rho[v]);
+        for (int v = 0; v < V; ++v) t[v] = f[v][q];
+        if (q != 0 && (nm_any & kNbrMoving)) {  // moving-wall neighbours are rare: one branch
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (((nmv[v] >> q) & 1u) && (nmv[v] & kNbrMoving)) {
+              const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
+              if (p.flags[m] & LBDEM_FLAG_MOVING) t[v] -= ladd_term<q, T>(p, m, rho[v]);
+            }
           }
         }
         // inactive cells' slots are never read, so they may receive garbage here
@@ -730,16 +757,19 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
       if (lane_in) {
         bool ok[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          ok[v] = act[v];
-          if (q != 0 && act[v] && ((nmv[v] >> q) & 1u)) {  // bounce back into the own opposite slot
-            T val = f[v][q];
-            if (nmv[v] & kNbrMoving) {
-              const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
-              if (p.flags[m] & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
+        for (int v = 0; v < V; ++v) ok[v] = act[v];
+        if (q != 0 && ((nm_any >> q) & 1u)) {  // a wall in direction q: one branch per direction
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (act[v] && ((nmv[v] >> q) & 1u)) {  // bounce back into the own opposite slot
+              T val = f[v][q];
+              if (nmv[v] & kNbrMoving) {
+                const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
+                if (p.flags[m] & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
+              }
+              Store<S>::st(F + kOPP[q] * N + base + k0 + v, val);
+              ok[v] = false;
             }
-            Store<S>::st(F + kOPP[q] * N + base + k0 + v, val);
-            ok[v] = false;
           }
         }
         if constexpr (cz == 0) {

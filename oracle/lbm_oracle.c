@@ -10,6 +10,16 @@
  *   - Smagorinsky uses the previous step's feq and omega (lagged), as the reference does.
  * REAL is double (the FP64 oracle, suffix _f64) or float (suffix _f32, used for CPU timing).
  * Every function cites the reference file:line it restates.
+ *
+ * How each part is pinned (the reference cannot run here: every module imports Taichi):
+ *   collide / stream / initialize: D3Q19 moment identities, mass conservation, Poiseuille and
+ *     shear-wave closed forms, lagged-vs-fixed-point Smagorinsky steady state (tests/);
+ *   psc_map / psc_forces: key rule, zero force in fluid at rest;
+ *   threedelta, ibm_spread, ibm_drag, ibm_weight, vans_spread, vans_drag: closed-form kernel
+ *     values, partition of unity, conservation of spread volume / momentum / force, dilute
+ *     (Schiller-Naumann) drag limits (tests/test_oracle_unresolved.py).
+ * Parity unpinned: tenneti_cd at finite solid fraction and Reynolds number, and the VANS
+ * forcing term F_q of vans_collide -- restatements only, no closed form or reference vector.
  */
 #include <math.h>
 #include <stdint.h>
