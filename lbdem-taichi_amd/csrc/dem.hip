@@ -707,15 +707,23 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     if (G.freeze) S.freeze = r_freeze;
     if (G.fixvel) S.fixvel = r_fixvel;
   }
+  // The hot broad-phase loops below read these views directly: in the resident kernel the
+  // compiler then knows they are LDS and issues LDS loads, instead of the generic (flat)
+  // accesses that the shared state view needs.
+  const double *posv = RESIDENT ? &r_vec[0][0] : G.pos;
+  const double *radv = RESIDENT ? r_rad : G.radius;
+  const int32_t *cfnv = RESIDENT ? r_cfn : G.cfn;
+  double *forcev = RESIDENT ? &r_vec[3][0] : G.force;
+  double *momv = RESIDENT ? &r_vec[6][0] : G.moment;
   for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;
   __syncthreads();
   for (int sub = 0; sub < nsub; ++sub) {
     // clear_state, own cell hash, push onto the bucket list
     for (int i = threadIdx.x; i < n; i += NT) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) S.force[3 * i + a] = S.moment[3 * i + a] = 0.0;
+      for (int a = 0; a < 3; ++a) forcev[3 * i + a] = momv[3 * i + a] = 0.0;
       int32_t c[3];
-      cell_of(P, S.pos + 3 * i, c);
+      cell_of(P, posv + 3 * i, c);
       const int32_t h = hash3(c[0], c[1], c[2], P.T);
       chash[i] = h;
       next[i] = atomicExch(&head[h & (kSmallSlots - 1)], i);
@@ -725,11 +733,19 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     // candidates of i; contacts already in a history slot are resolved here, new ones listed
     for (int i = threadIdx.x; i < n; i += NT) {
       int32_t h[27];
-      const int nc = probe_hashes(P, S.pos + 3 * i, h);
+      const double *xi = posv + 3 * i;
+      const int nc = probe_hashes(P, xi, h);
+      const double ri = radv[i];
+      const bool has_slots = cfnv[i] > 0;
       for (int k = 0; k < nc; ++k) {
         if (h[k] < 0) continue;
         for (int j = head[h[k] & (kSmallSlots - 1)]; j >= 0; j = next[j]) {
           if (j <= i || chash[j] != h[k]) continue;
+          if (!has_slots) {  // no history slot to update: only a touching pair needs work
+            double d[3];
+            periodic_delta(P, xi, posv + 3 * j, d);
+            if (!(norm3(d) - ri - radv[j] < 0)) continue;
+          }
           if (resolve_existing(P, S, i, j)) {
             const int slot = atomicAdd(&nnew, 1);
             if (slot < newcap) {
