@@ -233,7 +233,7 @@ def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=
                              damping_tangential=0.3, pp_friction=0.5, pw_friction=0.5)
     cfg = DEMSolverConfig(domain=DomainBounds(0.0, L[0], 0.0, L[1], 0.0, L[2]), dt=1.0 / subcycles,
                           gravity=(0.0, 0.0, -1e-5), contact_model=cm)
-    cfg.set_particle_properties(elastic_modulus=1.0, poisson_ratio=0.3, max_coordinate_number=16)
+    cfg.set_particle_properties(elastic_modulus=1.0, poisson_ratio=0.3, max_coordinate_number=32)
     cfg.set_wall_properties(elastic_modulus=1.0, poisson_ratio=0.3, density=2.5)
     dem = DEMSolver(cfg)
     dem.init_particle_arrays(radius=rad, mass=mass, position=pos, cell_size=float(d.max()), probe27=True)

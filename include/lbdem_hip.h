@@ -315,7 +315,7 @@ int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbd
 typedef struct lbdem_diff_args {
   int32_t nx, ny, nz;
   int32_t precision;        /* 32 or 64: element type of every state / adjoint / eps array */
-  const uint8_t *flags;     /* ncell flag bytes (ACTIVE, WALL) */
+  const uint8_t *flags;     /* ncell flag bytes (ACTIVE, WALL; NEAR from lbdem_lbm_prepare_flags) */
   const void *eps;          /* ncell solid fraction per cell */
   double tau;               /* relaxation time */
   double force[3];          /* body force, lattice units */

@@ -17,16 +17,16 @@ ARCH = os.environ.get("LBDEM_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 generation.  -ffp-contract=off keeps the float
 # arithmetic that must match the CPU restatement bit for bit (broad-phase cell keys, PSC
-# sub-cell coverage) free of fused multiply-adds.  The collide-stream translation units contract
-# within each source expression (-ffp-contract=on): the collision is HBM- and issue-bound, the
-# FMAs cut its instruction count by a fifth, and every kernel variant inlines the same collision
-# source, so the variants stay bit-identical to each other.
+# sub-cell coverage) free of fused multiply-adds.  The collide-stream and adjoint translation
+# units contract within each source expression (-ffp-contract=on): these kernels are HBM- and
+# issue-bound, the FMAs cut the collision's instruction count by a fifth, and every kernel
+# variant inlines the same collision source, so the variants stay bit-identical to each other.
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-CONTRACT = {"lbm_f32.hip", "lbm_f16.hip", "lbm_bf16.hip", "lbm_f64.hip"}
+CONTRACT = {"lbm_f32.hip", "lbm_f16.hip", "lbm_bf16.hip", "lbm_f64.hip", "adjoint.hip"}
 
 
 def _compile(src):

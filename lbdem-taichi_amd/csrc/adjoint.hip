@@ -86,35 +86,31 @@ __global__ __launch_bounds__(256) void k_diff_step(const DiffArgs a) {
                        uy, uz, B);
   const int nyz = a.ny * a.nz;
   const int i = c / nyz, j = (c - i * nyz) / a.nz, k = c - i * nyz - j * a.nz;
+  const bool near = fl & LBDEM_FLAG_NEAR;  // a wall among the neighbours (bounce-back)
   static_for<0, Q>([&](auto qc) {
     constexpr int q = qc;
     const int ii = wrapi(i + kCX[q], a.nx), jj = wrapi(j + kCY[q], a.ny), kk = wrapi(k + kCZ[q], a.nz);
     const int m = (ii * a.ny + jj) * a.nz + kk;
-    if (q != 0 && (a.flags[m] & kMaskSolid)) G[kOPP[q] * N + c] = g[q];  // bounce back
+    if (q != 0 && near && (a.flags[m] & kMaskSolid)) G[kOPP[q] * N + c] = g[q];  // bounce back
     else G[q * N + m] = g[q];
   });
 }
 
-__device__ __forceinline__ double block_sum_d(double v, double *sh) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wv] = v;
-  __syncthreads();
-  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
-}
-
+// Adjoint step.  Live state per cell: f and mu (2 x 19 values); feq and c.u are recomputed where
+// they are used instead of being kept, so the kernel stays below 128 VGPRs in FP32.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diff_adjoint(const DiffArgs a) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   double gt = 0.0, gfx = 0.0, gfy = 0.0, gfz = 0.0;
   const size_t N = (size_t)a.ncell;
-  if (c < a.ncell && (a.flags[c] & LBDEM_FLAG_ACTIVE)) {
+  const uint8_t fl = c < a.ncell ? a.flags[c] : (uint8_t)0;
+  if (fl & LBDEM_FLAG_ACTIVE) {
     const T *F = reinterpret_cast<const T *>(a.f_in);
     const T *L = reinterpret_cast<const T *>(a.lam_in);
     T *Lo = reinterpret_cast<T *>(a.lam_out);
     const int nyz = a.ny * a.nz;
     const int i = c / nyz, j = (c - i * nyz) / a.nz, k = c - i * nyz - j * a.nz;
+    const bool near = fl & LBDEM_FLAG_NEAR;  // a wall among the neighbours (bounce-back)
     // mu = S^T lambda^{n+1}
     T mu[Q], f[Q];
     static_for<0, Q>([&](auto qc) {
@@ -122,45 +118,77 @@ __global__ __launch_bounds__(256) void k_diff_adjoint(const DiffArgs a) {
       f[q] = F[q * N + c];
       const int ii = wrapi(i + kCX[q], a.nx), jj = wrapi(j + kCY[q], a.ny), kk = wrapi(k + kCZ[q], a.nz);
       const int m = (ii * a.ny + jj) * a.nz + kk;
-      if (q != 0 && (a.flags[m] & kMaskSolid)) mu[q] = L[kOPP[q] * N + c];
+      if (q != 0 && near && (a.flags[m] & kMaskSolid)) mu[q] = L[kOPP[q] * N + c];
       else mu[q] = L[q * N + m];
     });
+    T rho = T(0), jx = T(0), jy = T(0), jz = T(0);
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      rho += f[q];
+      acc<kCX[q]>(jx, f[q]);
+      acc<kCY[q]>(jy, f[q]);
+      acc<kCZ[q]>(jz, f[q]);
+    });
+    const T inv = recip(rho);
+    const T ux = jx * inv, uy = jy * inv, uz = jz * inv;
     const T eps = reinterpret_cast<const T *>(a.eps)[c];
     const T tau = T(a.tau), h = tau - T(0.5), omega = T(1) / tau;
+    const T B = eps * h / ((T(1) - eps) + h);
     const T Fx = T(a.Fx), Fy = T(a.Fy), Fz = T(a.Fz);
-    T g[Q], rho, ux, uy, uz, B;
-    psc_collide_fixed<T>(f, eps, tau, Fx, Fy, Fz, g, rho, ux, uy, uz, B);
     const T usq = ux * ux + uy * uy + uz * uz;
-    T feq[Q], cu[Q];
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      cu[q] = cdot<q>(ux, uy, uz);
-      feq[q] = T(kW[q]) * rho * (T(1) + T(3) * cu[q] + T(4.5) * cu[q] * cu[q] - T(1.5) * usq);
-    });
-    // nu = (dg/df)^T mu with g = a f + B P f + [(1-B) omega I - B P] feq + B w rho + (1-B) 3 rho w (c.F)
     const T oB = T(1) - B;
     const T ca = T(1) - B - oB * omega;
-    T xi[Q];
-    T sw = T(0), sF = T(0);
-    static_for<0, Q>([&](auto qc) {
+    const T oBw = oB * omega;
+    // nu = (dg/df)^T mu with g = a f + B P f + [(1-B) omega I - B P] feq + B w rho + (1-B) 3 rho w (c.F);
+    // J_feq^T xi = s0 + sum_a G_a (c_ka - u_a) with xi_q = (1-B) omega mu_q - B mu_qbar.
+    // Opposite pairs (q, qbar) share the even part E of feq (feq_q = E + O, feq_qbar = E - O).
+    T sw, sF = T(0), s0, X, Gx = T(0), Gy = T(0), Gz = T(0);  // X = sum_q w xi_q
+    T dB, dOm, dFx = T(0), dFy = T(0), dFz = T(0);
+    {
+      const T w = T(kW[0]);
+      const T fe = w * rho * (T(1) - T(1.5) * usq);
+      const T xi = (oBw - B) * mu[0];
+      sw = w * mu[0];
+      s0 = xi * fe;
+      X = w * xi;
+      dB = mu[0] * ((w * rho - fe) + omega * (f[0] - fe));
+      dOm = mu[0] * (-(oB) * (f[0] - fe));
+    }
+    static_for<1, Q>([&](auto qc) {
       constexpr int q = qc;
-      xi[q] = oB * omega * mu[q] - B * mu[kOPP[q]];
-      sw += T(kW[q]) * mu[q];
-      sF += T(kW[q]) * cdot<q>(Fx, Fy, Fz) * mu[q];
+      if constexpr (kOPP[q] > q) {
+        constexpr int r = kOPP[q];
+        const T w = T(kW[q]);
+        const T cu = cdot<q>(ux, uy, uz);
+        const T cF = cdot<q>(Fx, Fy, Fz);
+        const T E = w * rho * (T(1) + T(4.5) * cu * cu - T(1.5) * usq), O = T(3) * w * rho * cu;
+        const T fq = E + O, fr = E - O;
+        const T xq = oBw * mu[q] - B * mu[r], xr = oBw * mu[r] - B * mu[q];
+        const T xs = xq + xr, xd = xq - xr;
+        sw += w * (mu[q] + mu[r]);
+        sF += w * cF * (mu[q] - mu[r]);
+        s0 += xs * E + xd * O;
+        X += w * xs;
+        // G_a += w c_qa (3 (xi_q - xi_qbar) + 9 (c.u) (xi_q + xi_qbar))
+        const T gq = w * (T(3) * xd + T(9) * cu * xs);
+        acc<kCX[q]>(Gx, gq);
+        acc<kCY[q]>(Gy, gq);
+        acc<kCZ[q]>(Gz, gq);
+        // dg/dB = (P f - P feq + w rho - f) + omega (f - feq) - 3 rho w (c.F), for q and qbar
+        const T wr = w * rho, t3 = T(3) * wr * cF;
+        dB += mu[q] * ((f[r] - fr + wr - f[q]) + omega * (f[q] - fq) - t3) +
+              mu[r] * ((f[q] - fq + wr - f[r]) + omega * (f[r] - fr) + t3);
+        dOm += -(oB) * (mu[q] * (f[q] - fq) + mu[r] * (f[r] - fr));
+        const T sfd = oB * T(3) * wr * (mu[q] - mu[r]);
+        acc<kCX[q]>(dFx, sfd);
+        acc<kCY[q]>(dFy, sfd);
+        acc<kCZ[q]>(dFz, sfd);
+      }
     });
-    // J_feq^T xi: s0 + sum_a G_a (c_ka - u_a)
-    T s0 = T(0), Gx = T(0), Gy = T(0), Gz = T(0);
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      const T w = T(kW[q]);
-      s0 += xi[q] * feq[q];
-      const T t9 = T(9) * cu[q];
-      // G_a += xi_q w (3 c_qa + 9 (c.u) c_qa - 3 u_a)
-      Gx += xi[q] * w * ((kCX[q] ? T(kCX[q]) * (T(3) + t9) : T(0)) - T(3) * ux);
-      Gy += xi[q] * w * ((kCY[q] ? T(kCY[q]) * (T(3) + t9) : T(0)) - T(3) * uy);
-      Gz += xi[q] * w * ((kCZ[q] ? T(kCZ[q]) * (T(3) + t9) : T(0)) - T(3) * uz);
-    });
-    s0 /= rho;
+    Gx -= T(3) * ux * X;
+    Gy -= T(3) * uy * X;
+    Gz -= T(3) * uz * X;
+    s0 *= inv;
     const T common = s0 - (Gx * ux + Gy * uy + Gz * uz) + B * sw + oB * T(3) * sF;
     static_for<0, Q>([&](auto qc) {
       constexpr int q = qc;
@@ -169,21 +197,6 @@ __global__ __launch_bounds__(256) void k_diff_adjoint(const DiffArgs a) {
       acc<kCY[q]>(nu, Gy);
       acc<kCZ[q]>(nu, Gz);
       Lo[q * N + c] = nu;
-    });
-    // parameter gradients
-    T dB = T(0), dOm = T(0), dFx = T(0), dFy = T(0), dFz = T(0);
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      constexpr int r = kOPP[q];
-      const T w = T(kW[q]);
-      const T cF = cdot<q>(Fx, Fy, Fz);
-      // dg/dB = (P f - P feq + w rho - f) + omega (f - feq) - 3 rho w (c.F)
-      dB += mu[q] * ((f[r] - feq[r] + w * rho - f[q]) + omega * (f[q] - feq[q]) - T(3) * rho * w * cF);
-      dOm += mu[q] * (-(oB) * (f[q] - feq[q]));
-      const T s = mu[q] * oB * T(3) * rho * w;
-      acc<kCX[q]>(dFx, s);
-      acc<kCY[q]>(dFy, s);
-      acc<kCZ[q]>(dFz, s);
     });
     const T den = (T(1) - eps) + h;
     const T dBde = h * (T(1) + h) / (den * den);
@@ -194,16 +207,20 @@ __global__ __launch_bounds__(256) void k_diff_adjoint(const DiffArgs a) {
     gfy = (double)dFy;
     gfz = (double)dFz;
   }
-  __shared__ double sh[4];
-  const double r0 = block_sum_d(gt, sh);
-  const double r1 = block_sum_d(gfx, sh);
-  const double r2 = block_sum_d(gfy, sh);
-  const double r3 = block_sum_d(gfz, sh);
-  if (threadIdx.x == 0) {
-    unsafeAtomicAdd(a.g_red + 0, r0);
-    unsafeAtomicAdd(a.g_red + 1, r1);
-    unsafeAtomicAdd(a.g_red + 2, r2);
-    unsafeAtomicAdd(a.g_red + 3, r3);
+  // block sums in a fixed order (one barrier), then one FP64 atomic per value and block
+  __shared__ double sh[4][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double v0 = wave_sum(gt), v1 = wave_sum(gfx), v2 = wave_sum(gfy), v3 = wave_sum(gfz);
+  if (lane == 0) {
+    sh[0][wv] = v0;
+    sh[1][wv] = v1;
+    sh[2][wv] = v2;
+    sh[3][wv] = v3;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const double r = (sh[threadIdx.x][0] + sh[threadIdx.x][1]) + (sh[threadIdx.x][2] + sh[threadIdx.x][3]);
+    unsafeAtomicAdd(a.g_red + threadIdx.x, r);
   }
 }
 

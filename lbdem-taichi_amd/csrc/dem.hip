@@ -42,34 +42,36 @@ __device__ __forceinline__ void cell_of(const lbdem_dem_params &P, const double 
   for (int a = 0; a < 3; ++a) c[a] = (int32_t)floor((x[a] - P.dmin[a]) / P.cell);
 }
 
-// hashes of the probed cells of particle i: the octant's 8 cells (bpcd.py:114-152) or 27;
-// duplicates (hash aliasing) are replaced by -1 so every bucket is visited once.
-__device__ __forceinline__ int probe_hashes(const lbdem_dem_params &P, const double *x, int32_t *h) {
-  int32_t c[3];
-  cell_of(P, x, c);
-  int nc = 0;
-  if (!P.probe27) {
+// hashes of the probed cells of a particle in cell c at x: the octant's 8 cells
+// (bpcd.py:114-152) or all 27; duplicates (hash aliasing) become -1 so every bucket is visited
+// once.  NP is a template argument so every index into h is a compile-time constant.
+template <int NP>
+__device__ __forceinline__ void probe_cells(const lbdem_dem_params &P, const double *x, const int32_t (&c)[3],
+                                            int32_t (&h)[NP]) {
+  if constexpr (NP == 8) {
     int32_t d[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const double center = ((double)c[a] + 0.5) * P.cell + P.dmin[a];
       d[a] = (x[a] > center) ? 1 : -1;
     }
-    const int32_t o[8][3] = {{0, 0, 0},       {d[0], 0, 0},    {0, d[1], 0},    {0, 0, d[2]},
-                             {0, d[1], d[2]}, {d[0], 0, d[2]}, {d[0], d[1], 0}, {d[0], d[1], d[2]}};
-    for (int k = 0; k < 8; ++k) h[nc++] = hash3(c[0] + o[k][0], c[1] + o[k][1], c[2] + o[k][2], P.T);
+    h[0] = hash3(c[0], c[1], c[2], P.T);
+    h[1] = hash3(c[0] + d[0], c[1], c[2], P.T);
+    h[2] = hash3(c[0], c[1] + d[1], c[2], P.T);
+    h[3] = hash3(c[0], c[1], c[2] + d[2], P.T);
+    h[4] = hash3(c[0], c[1] + d[1], c[2] + d[2], P.T);
+    h[5] = hash3(c[0] + d[0], c[1], c[2] + d[2], P.T);
+    h[6] = hash3(c[0] + d[0], c[1] + d[1], c[2], P.T);
+    h[7] = hash3(c[0] + d[0], c[1] + d[1], c[2] + d[2], P.T);
   } else {
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -1; dz <= 1; ++dz) h[nc++] = hash3(c[0] + dx, c[1] + dy, c[2] + dz, P.T);
+#pragma unroll
+    for (int t = 0; t < 27; ++t) h[t] = hash3(c[0] + t / 9 - 1, c[1] + (t / 3) % 3 - 1, c[2] + t % 3 - 1, P.T);
   }
-  for (int k = 1; k < nc; ++k)
+#pragma unroll
+  for (int k = 1; k < NP; ++k)
+#pragma unroll
     for (int e = 0; e < k; ++e)
-      if (h[e] == h[k]) {
-        h[k] = -1;
-        break;
-      }
-  return nc;
+      if (h[e] == h[k]) h[k] = -1;  // a later match compares against -1 only, which changes nothing
 }
 
 __global__ __launch_bounds__(256) void k_bp_count(const lbdem_dem_params P, const double *pos, int32_t *hash,
@@ -147,15 +149,19 @@ __global__ __launch_bounds__(256) void k_bp_scatter(const lbdem_dem_params P, co
 }
 
 // probe: candidate pairs (i, pid > i); one atomic per wavefront reserves the output range
+template <int NP>
 __global__ __launch_bounds__(256) void k_bp_probe(const lbdem_dem_params P, const double *pos,
                                                   const int32_t *count, const int32_t *offset, const int32_t *pid,
                                                   int32_t *pairs, int64_t cap, unsigned long long *npairs) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  int32_t h[27];
-  int nc = 0;
+  int32_t h[NP];
+  constexpr int nc = NP;
   int mine = 0;
   if (i < P.n) {
-    nc = probe_hashes(P, pos + 3 * i, h);
+    int32_t c[3];
+    cell_of(P, pos + 3 * i, c);
+    probe_cells<NP>(P, pos + 3 * i, c, h);
+#pragma unroll
     for (int k = 0; k < nc; ++k) {
       if (h[k] < 0) continue;
       const int32_t b = offset[h[k]], e = b + count[h[k]];
@@ -176,6 +182,7 @@ __global__ __launch_bounds__(256) void k_bp_probe(const lbdem_dem_params P, cons
   base = __shfl(base, 63, 64);
   unsigned long long w = base + (unsigned long long)(incl - mine);
   if (i < P.n && mine > 0) {
+#pragma unroll
     for (int k = 0; k < nc; ++k) {
       if (h[k] < 0) continue;
       const int32_t b = offset[h[k]], e = b + count[h[k]];
@@ -658,6 +665,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   __shared__ int32_t head[kSmallSlots];
   __shared__ int32_t next[kSmallN];
   __shared__ int32_t chash[kSmallN];
+  __shared__ int32_t ccell[3 * (RESIDENT ? kResidentN : kSmallN)];  // cell coordinates of phase A
   __shared__ int nnew;
   const int n = P.n;
   lbdem_dem_state S = G;
@@ -726,36 +734,46 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
       cell_of(P, posv + 3 * i, c);
       const int32_t h = hash3(c[0], c[1], c[2], P.T);
       chash[i] = h;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) ccell[3 * i + a] = c[a];
       next[i] = atomicExch(&head[h & (kSmallSlots - 1)], i);
     }
     if (threadIdx.x == 0) nnew = 0;
     __syncthreads();
     // candidates of i; contacts already in a history slot are resolved here, new ones listed
-    for (int i = threadIdx.x; i < n; i += NT) {
-      int32_t h[27];
-      const double *xi = posv + 3 * i;
-      const int nc = probe_hashes(P, xi, h);
-      const double ri = radv[i];
-      const bool has_slots = cfnv[i] > 0;
-      for (int k = 0; k < nc; ++k) {
-        if (h[k] < 0) continue;
-        for (int j = head[h[k] & (kSmallSlots - 1)]; j >= 0; j = next[j]) {
-          if (j <= i || chash[j] != h[k]) continue;
-          if (!has_slots) {  // no history slot to update: only a touching pair needs work
-            double d[3];
-            periodic_delta(P, xi, posv + 3 * j, d);
-            if (!(norm3(d) - ri - radv[j] < 0)) continue;
-          }
-          if (resolve_existing(P, S, i, j)) {
-            const int slot = atomicAdd(&nnew, 1);
-            if (slot < newcap) {
-              newlist[2 * slot] = i;
-              newlist[2 * slot + 1] = j;
+    // (the probe count is a template argument of the walk so the hash array stays in registers)
+    auto walk = [&](auto np) {
+      constexpr int NP = decltype(np)::value;
+      for (int i = threadIdx.x; i < n; i += NT) {
+        const double *xi = posv + 3 * i;
+        const int32_t ci[3] = {ccell[3 * i], ccell[3 * i + 1], ccell[3 * i + 2]};
+        int32_t h[NP];
+        probe_cells<NP>(P, xi, ci, h);
+        const double ri = radv[i];
+        const bool has_slots = cfnv[i] > 0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          if (h[k] < 0) continue;
+          for (int j = head[h[k] & (kSmallSlots - 1)]; j >= 0; j = next[j]) {
+            if (j <= i || chash[j] != h[k]) continue;
+            if (!has_slots) {  // no history slot to update: only a touching pair needs work
+              double d[3];
+              periodic_delta(P, xi, posv + 3 * j, d);
+              if (!(norm3(d) - ri - radv[j] < 0)) continue;
+            }
+            if (resolve_existing(P, S, i, j)) {
+              const int slot = atomicAdd(&nnew, 1);
+              if (slot < newcap) {
+                newlist[2 * slot] = i;
+                newlist[2 * slot + 1] = j;
+              }
             }
           }
         }
       }
-    }
+    };
+    if (P.probe27) walk(std::integral_constant<int, 27>{});
+    else walk(std::integral_constant<int, 8>{});
     __syncthreads();
     const int nn = min(nnew, newcap);
     if (threadIdx.x == 0 && nnew > newcap) atomicAdd(S.status + 0, nnew - newcap);
@@ -800,8 +818,12 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, ntiles);
   hipLaunchKernelGGL(k_scan_add, dim3((P->T + 255) / 256), dim3(256), 0, s, w->offset, w->tile_sum, P->T);
   hipLaunchKernelGGL(k_bp_scatter, dim3(nb), dim3(256), 0, s, *P, w->hash, w->offset, w->current, w->pid);
-  hipLaunchKernelGGL(k_bp_probe, dim3(nb), dim3(256), 0, s, *P, pos, w->count, w->offset, w->pid, w->pairs,
-                     w->pair_cap, w->npairs);
+  if (P->probe27)
+    hipLaunchKernelGGL(k_bp_probe<27>, dim3(nb), dim3(256), 0, s, *P, pos, w->count, w->offset, w->pid, w->pairs,
+                       w->pair_cap, w->npairs);
+  else
+    hipLaunchKernelGGL(k_bp_probe<8>, dim3(nb), dim3(256), 0, s, *P, pos, w->count, w->offset, w->pid, w->pairs,
+                       w->pair_cap, w->npairs);
   hipLaunchKernelGGL(k_bp_reset, dim3(nb), dim3(256), 0, s, *P, w->hash, w->count, w->current);
   LBDEM_CHECK_LAUNCH(who);
   return LBDEM_OK;

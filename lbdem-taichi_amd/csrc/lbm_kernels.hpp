@@ -630,23 +630,19 @@ struct Vec {
   }
 };
 
-// LDS row of a z-moving direction: cell k at index k + 1, with halo slots 0 (cell nz - 1) and
-// nz + 1 (cell 0), so the +-1 shifted reads need no wrap arithmetic.  Row stride: ncz + 2.
 template <typename S, int COLL, int COUP, bool ODD, int V>
 __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
   using T = typename Store<S>::T;
   extern __shared__ __align__(16) unsigned char smem[];
+  T *buf = reinterpret_cast<T *>(smem);  // [kNZ][ncz]
   const int nz = p.nz;
-  const int ncz = blockDim.x * V;  // row cells covered by the block (>= nz)
-  const int rs = ncz + 2;          // LDS row stride (halo on both ends)
-  T *buf = reinterpret_cast<T *>(smem);                     // [kNZ][rs] staged populations
-  unsigned char *okb = smem + sizeof(T) * kNZ * rs;         // [kNZ][rs] push validity
+  const int ncz = blockDim.x * V;         // row cells covered by the block (>= nz)
+  unsigned char *okb = smem + sizeof(T) * kNZ * ncz;  // [kNZ][ncz] push validity
   const int row = blockIdx.x;
   const int i = row / p.ny, j = row - (row / p.ny) * p.ny;
   const int k0 = threadIdx.x * V;
   const unsigned uk0 = threadIdx.x * V;  // per-lane part of every row address (uniform row base)
   const bool lane_in = k0 < nz;
-  const bool first = k0 == 0, last = k0 + V == nz;  // owners of the row ends (halo writers)
   S *__restrict__ F = reinterpret_cast<S *>(p.f);
   const size_t N = (size_t)p.ncell;
   const int base = row * nz;
@@ -660,15 +656,12 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
   }
   uint32_t nmv[V];  // wall-neighbour bits (NEAR cells only)
   uint32_t nm_any = 0u;  // union over this thread's cells: the wall paths below branch on it
-  bool all_act = true;
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     nmv[v] = (fl[v] & LBDEM_FLAG_NEAR) ? p.nmask[base + k0 + v] : 0u;
     nm_any |= nmv[v];
-    all_act = all_act && act[v];
   }
   T f[V][Q];
-  int plain = 0;  // odd steps: every cell of the row active and no wall neighbour (block-uniform)
 
   // ---------------------------------------------------------------- gather
   if constexpr (!ODD) {
@@ -693,36 +686,34 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
 #pragma unroll
           for (int v = 0; v < V; ++v) f[v][q] = t[v];
         } else {
-          T *b = buf + zslot<q>() * rs;
 #pragma unroll
-          for (int v = 0; v < V; ++v) b[k0 + v + 1] = t[v];
-          if (first) b[nz + 1] = t[0];
-          if (last) b[0] = t[V - 1];
+          for (int v = 0; v < V; ++v) buf[zslot<q>() * ncz + k0 + v] = t[v];
         }
       }
     });
-    plain = __syncthreads_and(lane_in ? (all_act && nm_any == 0u) : 1);
+    __syncthreads();
     if (lane_in) {
       static_for<1, Q>([&](auto qc) {
         constexpr int q = qc;
         constexpr int cz = kCZ[q];
         if constexpr (cz != 0) {
-          const T *b = buf + zslot<q>() * rs + (1 - cz);  // cell k0 + v - cz
 #pragma unroll
-          for (int v = 0; v < V; ++v) f[v][q] = b[k0 + v];
+          for (int v = 0; v < V; ++v) {
+            int ks = k0 + v - cz;
+            ks = ks < 0 ? ks + nz : (ks >= nz ? ks - nz : ks);
+            f[v][q] = buf[zslot<q>() * ncz + ks];
+          }
         }
       });
-      if (nm_any) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          if (fl[v] & LBDEM_FLAG_NEAR) {  // populations bounced back at a wall last step
-            const int k = k0 + v;
-            const uint32_t nm = nmv[v];
-            static_for<1, Q>([&](auto qc) {
-              constexpr int q = qc;
-              if ((nm >> kOPP[q]) & 1u) f[v][q] = Store<S>::ld(F + q * N + base + k);
-            });
-          }
+      for (int v = 0; v < V; ++v) {
+        if (fl[v] & LBDEM_FLAG_NEAR) {  // populations bounced back at a wall last step
+          const int k = k0 + v;
+          const uint32_t nm = nmv[v];
+          static_for<1, Q>([&](auto qc) {
+            constexpr int q = qc;
+            if ((nm >> kOPP[q]) & 1u) f[v][q] = Store<S>::ld(F + q * N + base + k);
+          });
         }
       }
     }
@@ -786,28 +777,21 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
           bool all = true;
 #pragma unroll
           for (int v = 0; v < V; ++v) all = all && ok[v];
-          T t[V];
+          if (all) {
+            T t[V];
 #pragma unroll
-          for (int v = 0; v < V; ++v) t[v] = f[v][q];
-          if (plain || all) {
+            for (int v = 0; v < V; ++v) t[v] = f[v][q];
             Vec<S, V>::st(dst, t);
           } else {
 #pragma unroll
             for (int v = 0; v < V; ++v)
-              if (ok[v]) Store<S>::st(dst + v, t[v]);
+              if (ok[v]) Store<S>::st(dst + v, f[v][q]);
           }
         } else {
-          T *b = buf + zslot<q>() * rs;
 #pragma unroll
-          for (int v = 0; v < V; ++v) b[k0 + v + 1] = f[v][q];
-          if (first) b[nz + 1] = f[0][q];
-          if (last) b[0] = f[V - 1][q];
-          if (!plain) {
-            unsigned char *o = okb + zslot<q>() * rs;
-#pragma unroll
-            for (int v = 0; v < V; ++v) o[k0 + v + 1] = ok[v] ? 1 : 0;
-            if (first) o[nz + 1] = ok[0] ? 1 : 0;
-            if (last) o[0] = ok[V - 1] ? 1 : 0;
+          for (int v = 0; v < V; ++v) {
+            buf[zslot<q>() * ncz + k0 + v] = f[v][q];
+            okb[zslot<q>() * ncz + k0 + v] = ok[v] ? 1 : 0;
           }
         }
       }
@@ -820,27 +804,22 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
         if constexpr (cz != 0) {
           const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
           S *dst = F + (q * N + (size_t)trow * nz) + uk0;
-          const T *b = buf + zslot<q>() * rs + (1 - cz);  // the cell pushing into k0 + v: k0 + v - cz
           T t[V];
+          bool ok[V], all = true;
 #pragma unroll
-          for (int v = 0; v < V; ++v) t[v] = b[k0 + v];
-          if (plain) {
+          for (int v = 0; v < V; ++v) {
+            int ks = k0 + v - cz;  // the cell pushing into (row trow, k0 + v)
+            ks = ks < 0 ? ks + nz : (ks >= nz ? ks - nz : ks);
+            t[v] = buf[zslot<q>() * ncz + ks];
+            ok[v] = okb[zslot<q>() * ncz + ks] != 0;
+            all = all && ok[v];
+          }
+          if (all) {
             Vec<S, V>::st(dst, t);
           } else {
-            const unsigned char *o = okb + zslot<q>() * rs + (1 - cz);
-            bool ok[V], all = true;
 #pragma unroll
-            for (int v = 0; v < V; ++v) {
-              ok[v] = o[k0 + v] != 0;
-              all = all && ok[v];
-            }
-            if (all) {
-              Vec<S, V>::st(dst, t);
-            } else {
-#pragma unroll
-              for (int v = 0; v < V; ++v)
-                if (ok[v]) Store<S>::st(dst + v, t[v]);
-            }
+            for (int v = 0; v < V; ++v)
+              if (ok[v]) Store<S>::st(dst + v, t[v]);
           }
         }
       });

@@ -57,6 +57,12 @@ class DiffPSCChannel:
         flags = np.where(kind == CellType.FLUID, H.FLAG_ACTIVE | H.FLAG_FLUID, H.FLAG_WALL).astype(np.uint8)
         self.device = torch.device("cuda")
         self.flags = torch.from_numpy(flags.reshape(-1)).to(self.device)
+        # NEAR bits (cells with a wall neighbour take the bounce-back path) derived on the device
+        dims = H.Lattice()
+        dims.nx, dims.ny, dims.nz = self.nx, self.ny, self.nz
+        nmask = torch.zeros(self.N, dtype=torch.int32, device=self.device)
+        H.check(H.lib().lbdem_lbm_prepare_flags(ctypes.byref(dims), H.ptr(self.flags), H.ptr(nmask), H.stream()),
+                "lbdem_lbm_prepare_flags")
         self._args = DiffArgs()
         a = self._args
         a.nx, a.ny, a.nz = self.nx, self.ny, self.nz
