@@ -656,8 +656,20 @@ __global__ __launch_bounds__(256) void k_dem_compact(const lbdem_dem_params P, c
 // lives in LDS for the whole launch: the state view's pointers are redirected to LDS copies (flat addressing), so the
 // contact, wall and integration code is shared with the other paths; contact history stays in
 // global memory.
+// The resident kernel also keeps a Verlet list: each particle's j > i within
+// r_i + r_j + skin, found with the 27-cell probe (which covers every distance below one cell),
+// skin = min(cell - 2 r_max, r_max).  The list is rebuilt when some particle has moved more than
+// skin / 2 since the last build, so every pair that touches, and every pair holding a history
+// slot, is in the list: the contacts found are those of the per-substep hash probe.
 constexpr int kSmallSlots = 4096;
 constexpr int kResidentN = 256;
+constexpr int kNbr = 16;  // Verlet neighbours kept per particle; more -> per-substep probe instead
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
 
 template <bool RESIDENT, int NT>
 __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, const lbdem_dem_state G, int nsub,
@@ -679,6 +691,14 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   __shared__ double r_inertia[9 * kRes];
   __shared__ double r_fluid[2][3 * kRes];
   __shared__ int32_t r_freeze[kRes], r_fixvel[kRes];
+  __shared__ int32_t nb_cnt[kRes], nb_list[kNbr * kRes];  // Verlet lists (resident kernel)
+  __shared__ double x_ref[3 * kRes];                       // positions at the last list build
+  __shared__ double vl_skin;
+  // list state; each is written only in a phase where no thread reads it before the next barrier:
+  // vl_rebuild (set in the integration phase, cleared by thread 0 after the pair phase),
+  // vl_off (lists unusable for this launch: no skin or a list overflowed), vl_over (build phase)
+  __shared__ int vl_rebuild, vl_off, vl_over;
+  __shared__ double rmax_w[NT / 64];
   if constexpr (RESIDENT) {
     double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
     for (int t = threadIdx.x; t < 3 * n; t += NT) {
@@ -724,6 +744,21 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   double *forcev = RESIDENT ? &r_vec[3][0] : G.force;
   double *momv = RESIDENT ? &r_vec[6][0] : G.moment;
   for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;
+  if constexpr (RESIDENT) {
+    double rm = 0.0;
+    for (int t = threadIdx.x; t < n; t += NT) rm = fmax(rm, r_rad[t]);
+    rm = wave_max(rm);
+    if ((threadIdx.x & 63) == 0) rmax_w[threadIdx.x >> 6] = rm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r = 0.0;
+      for (int w = 0; w < NT / 64; ++w) r = fmax(r, rmax_w[w]);
+      vl_skin = fmin(P.cell - 2.0 * r, r);
+      vl_rebuild = 1;
+      vl_off = vl_skin > 0.0 ? 0 : 1;
+      vl_over = 0;
+    }
+  }
   __syncthreads();
   for (int sub = 0; sub < nsub; ++sub) {
     // clear_state, own cell hash, push onto the bucket list
@@ -772,9 +807,76 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
         }
       }
     };
-    if (P.probe27) walk(std::integral_constant<int, 27>{});
-    else walk(std::integral_constant<int, 8>{});
+    // one pair (i, j > i) of a candidate list: touching pairs and pairs with history are resolved
+    auto visit = [&](int i, const double *xi, double ri, bool has_slots, int j) {
+      if (!has_slots) {  // no history slot to update: only a touching pair needs work
+        double d[3];
+        periodic_delta(P, xi, posv + 3 * j, d);
+        if (!(norm3(d) - ri - radv[j] < 0)) return;
+      }
+      if (resolve_existing(P, S, i, j)) {
+        const int slot = atomicAdd(&nnew, 1);
+        if (slot < newcap) {
+          newlist[2 * slot] = i;
+          newlist[2 * slot + 1] = j;
+        }
+      }
+    };
+    bool use_lists = false;
+    bool rebuilt = false;
+    if constexpr (RESIDENT) {
+      rebuilt = vl_rebuild && !vl_off;  // uniform: neither changes during this phase
+      if (rebuilt) {  // (re)build the Verlet lists with the 27-cell probe
+        const double skin = vl_skin;
+        for (int i = threadIdx.x; i < n; i += NT) {
+          const double *xi = posv + 3 * i;
+          const int32_t ci[3] = {ccell[3 * i], ccell[3 * i + 1], ccell[3 * i + 2]};
+          const double ri = radv[i];
+          int cnt = 0;
+          // the 27 cells around i; j is taken only from the list walk of its own cell, so a
+          // bucket shared by several cells (hash aliasing) never yields a pair twice
+          for (int t = 0; t < 27; ++t) {
+            const int32_t cx = ci[0] + t / 9 - 1, cy = ci[1] + (t / 3) % 3 - 1, cz = ci[2] + t % 3 - 1;
+            const int32_t hc = hash3(cx, cy, cz, P.T);
+            for (int j = head[hc & (kSmallSlots - 1)]; j >= 0; j = next[j]) {
+              if (j <= i || ccell[3 * j] != cx || ccell[3 * j + 1] != cy || ccell[3 * j + 2] != cz) continue;
+              double d[3];
+              periodic_delta(P, xi, posv + 3 * j, d);
+              if (norm3(d) - ri - radv[j] < skin) {
+                if (cnt < kNbr) nb_list[kNbr * i + cnt] = j;
+                ++cnt;
+              }
+            }
+          }
+          nb_cnt[i] = cnt;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) x_ref[3 * i + a] = xi[a];
+          if (cnt > kNbr) atomicOr(&vl_over, 1);
+        }
+        __syncthreads();
+      }
+      use_lists = !vl_off && !vl_over;  // vl_over is final after the build barrier
+    }
+    if (use_lists) {
+      for (int i = threadIdx.x; i < n; i += NT) {
+        const double *xi = posv + 3 * i;
+        const double ri = radv[i];
+        const bool has_slots = cfnv[i] > 0;
+        const int cnt = nb_cnt[i];
+        for (int m = 0; m < cnt; ++m) visit(i, xi, ri, has_slots, nb_list[kNbr * i + m]);
+      }
+    } else if (P.probe27) {
+      walk(std::integral_constant<int, 27>{});
+    } else {
+      walk(std::integral_constant<int, 8>{});
+    }
     __syncthreads();
+    if constexpr (RESIDENT) {
+      if (threadIdx.x == 0 && rebuilt) {
+        vl_rebuild = 0;  // lists are current
+        if (vl_over) vl_off = 1;
+      }
+    }
     const int nn = min(nnew, newcap);
     if (threadIdx.x == 0 && nnew > newcap) atomicAdd(S.status + 0, nnew - newcap);
     for (int t = threadIdx.x; t < nn; t += NT) resolve_new(P, S, newlist[2 * t], newlist[2 * t + 1]);
@@ -784,6 +886,14 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     for (int i = threadIdx.x; i < n; i += NT) {
       walls_and_integrate(P, S, i);
       compact_slots(P, S, i);
+      if constexpr (RESIDENT) {
+        if (!vl_off) {  // moved more than half the skin since the build: rebuild next substep
+          double d[3];
+          periodic_delta(P, x_ref + 3 * i, posv + 3 * i, d);
+          const double hs = 0.5 * vl_skin;
+          if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > hs * hs) atomicOr(&vl_rebuild, 1);
+        }
+      }
     }
     __syncthreads();
   }

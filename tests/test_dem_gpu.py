@@ -103,6 +103,44 @@ def test_dem_small_path_state_in_global_memory():
     assert gpu.status()["slot_overflow"] == 0
 
 
+def test_dem_resident_path_neighbour_list_overflow():
+    """One large sphere among 150 small ones: the large sphere has far more than the 16 Verlet
+    neighbours the single-workgroup kernel keeps, so it falls back to the per-substep hash
+    probe; the trajectories and the contact set still match the oracle."""
+    rng = np.random.default_rng(12)
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    cfg = make_config(model="hertz", box=box, dt=5e-5, g=(0.0, 0.0, 0.0))
+    pos, rad = [np.array([0.5, 0.5, 0.5])], [0.12]
+    while len(pos) < 151:
+        x = rng.uniform(0.03, 0.97, 3)
+        r = 0.02
+        if (np.linalg.norm(np.array(pos) - x, axis=1) - np.array(rad) - r < 0.002).any():
+            continue
+        if np.linalg.norm(x - pos[0]) > 0.2:  # crowd the small spheres round the large one
+            continue
+        pos.append(x)
+        rad.append(r)
+    pos, rad = np.array(pos), np.array(rad)
+    mass = 2500.0 * 4.0 / 3.0 * np.pi * rad ** 3
+    vel = rng.uniform(-0.3, 0.3, pos.shape)
+    vel[1:] -= 0.5 * (pos[1:] - pos[0])  # drift towards the large sphere: contacts form
+    gpu = gpu_solver(cfg, rad, mass, pos, vel, path=1)
+    orc = oracle_solver(cfg, rad, mass, pos, vel)
+    for _ in range(6):
+        gpu.run_substeps(50)
+        orc.step(50)
+    torch.cuda.synchronize()
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+    assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
+    K = gpu.K
+    act = gpu._cf_active.cpu().numpy().astype(bool)
+    gi, gj = np.repeat(np.arange(gpu.n), K)[act], gpu._cf_j.cpu().numpy()[act]
+    oact = orc["cf_active"].astype(bool)
+    oi, oj = np.repeat(np.arange(orc.n), K)[oact], orc["cf_j"][oact]
+    assert len(gi) > 0, "the case must make contacts"
+    assert sorted(zip(gi.tolist(), gj.tolist())) == sorted(zip(oi.tolist(), oj.tolist()))
+
+
 def test_dem_small_and_general_paths_agree():
     a, _ = _run_pair("hertz", 1, seed=9)
     b, _ = _run_pair("hertz", 2, seed=9)
