@@ -287,6 +287,10 @@ typedef struct lbdem_dem_work {
   int64_t pair_cap;
   unsigned long long *npairs;        /* 1: candidate count (may exceed pair_cap) */
   int32_t *need_new;                 /* pair_cap */
+  int32_t *verlet;                   /* 2 + 17n, may be NULL: neighbour lists the single-workgroup
+                                        path keeps between calls (n <= 256); zero it, or its first
+                                        element, whenever radii or contact history change */
+  double *verlet_ref;                /* 1 + 3n, may be NULL (with verlet) */
 } lbdem_dem_work;
 
 /* Candidate pairs of the Morton-hash broad phase: BPCD.detect_collision with the count,

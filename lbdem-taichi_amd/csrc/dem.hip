@@ -673,7 +673,7 @@ __device__ __forceinline__ double wave_max(double v) {
 
 template <bool RESIDENT, int NT>
 __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, const lbdem_dem_state G, int nsub,
-                                                    int32_t *newlist, int newcap) {
+                                                    int32_t *newlist, int newcap, int32_t *vl_g, double *vl_xg) {
   __shared__ int32_t head[kSmallSlots];
   __shared__ int32_t next[kSmallN];
   __shared__ int32_t chash[kSmallN];
@@ -697,7 +697,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   // list state; each is written only in a phase where no thread reads it before the next barrier:
   // vl_rebuild (set in the integration phase, cleared by thread 0 after the pair phase),
   // vl_off (lists unusable for this launch: no skin or a list overflowed), vl_over (build phase)
-  __shared__ int vl_rebuild, vl_off, vl_over;
+  __shared__ int vl_rebuild, vl_off, vl_over, vl_load;
   __shared__ double rmax_w[NT / 64];
   if constexpr (RESIDENT) {
     double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
@@ -754,17 +754,37 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
       double r = 0.0;
       for (int w = 0; w < NT / 64; ++w) r = fmax(r, rmax_w[w]);
       vl_skin = fmin(P.cell - 2.0 * r, r);
-      vl_rebuild = 1;
       vl_off = vl_skin > 0.0 ? 0 : 1;
       vl_over = 0;
+      // lists kept by the previous launch: valid flag, particle count and skin must match
+      vl_load = (!vl_off && vl_g && vl_g[0] == 1 && vl_g[1] == n && vl_xg[0] == vl_skin) ? 1 : 0;
+      vl_rebuild = vl_load ? 0 : 1;
+    }
+    __syncthreads();
+    if (vl_load) {
+      for (int t = threadIdx.x; t < n; t += NT) nb_cnt[t] = vl_g[2 + t];
+      for (int t = threadIdx.x; t < kNbr * n; t += NT) nb_list[t] = vl_g[2 + n + t];
+      for (int t = threadIdx.x; t < 3 * n; t += NT) x_ref[t] = vl_xg[1 + t];
+      __syncthreads();
+      // positions changed between launches (by the caller) by more than half the skin: rebuild
+      for (int i = threadIdx.x; i < n; i += NT) {
+        double d[3];
+        periodic_delta(P, x_ref + 3 * i, posv + 3 * i, d);
+        const double hs = 0.5 * vl_skin;
+        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > hs * hs) atomicOr(&vl_rebuild, 1);
+      }
     }
   }
   __syncthreads();
   for (int sub = 0; sub < nsub; ++sub) {
+    // bucket lists are needed by a Verlet-list build and by the per-substep probe
+    bool pushed = true;
+    if constexpr (RESIDENT) pushed = vl_rebuild || vl_off;  // uniform: stable in this phase
     // clear_state, own cell hash, push onto the bucket list
     for (int i = threadIdx.x; i < n; i += NT) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) forcev[3 * i + a] = momv[3 * i + a] = 0.0;
+      if (!pushed) continue;
       int32_t c[3];
       cell_of(P, posv + 3 * i, c);
       const int32_t h = hash3(c[0], c[1], c[2], P.T);
@@ -880,7 +900,8 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     const int nn = min(nnew, newcap);
     if (threadIdx.x == 0 && nnew > newcap) atomicAdd(S.status + 0, nnew - newcap);
     for (int t = threadIdx.x; t < nn; t += NT) resolve_new(P, S, newlist[2 * t], newlist[2 * t + 1]);
-    for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;  // lists are rebuilt next substep
+    if (pushed)
+      for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;  // lists are rebuilt next substep
     __syncthreads();
     // integration and slot compaction touch particle i's own data only
     for (int i = threadIdx.x; i < n; i += NT) {
@@ -905,6 +926,19 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     }
     for (int t = threadIdx.x; t < 4 * n; t += NT) G.quat[t] = r_quat[t];
     for (int t = threadIdx.x; t < n; t += NT) G.cfn[t] = r_cfn[t];
+    if (vl_g) {  // keep the Verlet lists for the next launch
+      const bool keep = !vl_off && !vl_rebuild;
+      if (keep) {
+        for (int t = threadIdx.x; t < n; t += NT) vl_g[2 + t] = nb_cnt[t];
+        for (int t = threadIdx.x; t < kNbr * n; t += NT) vl_g[2 + n + t] = nb_list[t];
+        for (int t = threadIdx.x; t < 3 * n; t += NT) vl_xg[1 + t] = x_ref[t];
+      }
+      if (threadIdx.x == 0) {
+        vl_g[0] = keep ? 1 : 0;
+        vl_g[1] = n;
+        vl_xg[0] = vl_skin;
+      }
+    }
   }
 }
 
@@ -973,9 +1007,10 @@ int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbd
     const int newcap = (int)min(w->pair_cap, (int64_t)0x7fffffff);
     if (n <= kResidentN)
       hipLaunchKernelGGL((k_dem_small<true, kResidentN>), dim3(1), dim3(kResidentN), 0, s, *P, *S, nsub, w->pairs,
-                         newcap);
+                         newcap, w->verlet, w->verlet_ref);
     else
-      hipLaunchKernelGGL((k_dem_small<false, 512>), dim3(1), dim3(512), 0, s, *P, *S, nsub, w->pairs, newcap);
+      hipLaunchKernelGGL((k_dem_small<false, 512>), dim3(1), dim3(512), 0, s, *P, *S, nsub, w->pairs, newcap,
+                         nullptr, nullptr);
     LBDEM_CHECK_LAUNCH("lbdem_dem_run(small)");
     return LBDEM_OK;
   }

@@ -267,6 +267,10 @@ class DEMSolver:
         self.bpcd = BPCD.create(n, max_radius, dmin, dmax, cell_size=cell_size, probe27=probe27)
         self._ws = None
         self._pair_cap = max(256 * n, 4096)
+        # neighbour lists the single-workgroup kernel keeps between calls (first element: valid)
+        self._verlet = torch.zeros(2 + 17 * n, dtype=torch.int32, device=self.device)
+        self._verlet_ref = torch.zeros(1 + 3 * n, dtype=torch.float64, device=self.device)
+        self._radius_version = None
 
     def set_contact_model(self, model_type: str = "hertz"):
         """Select the contact law; must match the config's contact model (demsolver.py:311-347)."""
@@ -340,13 +344,21 @@ class DEMSolver:
         if cached is None:
             P = self._params()
             S = self._state()
-            _, w = self.bpcd.workspace(self.n, self._pair_cap, self.device)
-            cached = self._cached = (P, S, w, ctypes.byref(P), ctypes.byref(S), ctypes.byref(w))
+            # the workspace tensors are held here too, so a later broad-phase call that resizes the
+            # BPCD workspace cannot free memory this solver's kernels still point at
+            t, w = self.bpcd.workspace(self.n, self._pair_cap, self.device)
+            w.verlet = self._verlet.data_ptr()
+            w.verlet_ref = self._verlet_ref.data_ptr()
+            cached = self._cached = (P, S, w, ctypes.byref(P), ctypes.byref(S), ctypes.byref(w), t)
+            self._verlet[:1].zero_()
         else:
             self.gf.sync()
+        if self.gf.radius.version != self._radius_version:  # radii written since the last call
+            self._verlet[:1].zero_()
         _hip.check(_hip.lib().lbdem_dem_run(cached[3], cached[4], cached[5], int(nsub), int(self.path),
                                             _hip.stream()), "lbdem_dem_run")
         self.gf.touch()
+        self._radius_version = self.gf.radius.version
 
     def run_simulation(self):
         """One DEM step (demsolver.py:848-862)."""
