@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 2
+#define LBDEM_ABI_VERSION 3
 
 /* return codes */
 #define LBDEM_OK 0
@@ -57,7 +57,9 @@ extern "C" {
 #define LBDEM_COUPLE_VANS 3  /* volume-averaged NS with Guo-type force */
 
 /* per-cell flag byte */
-#define LBDEM_FLAG_ACTIVE 0x01 /* collided and streamed (not OBSTACLE / VEL_LADD) */
+#define LBDEM_FLAG_ACTIVE 0x01 /* collided and streamed (not OBSTACLE / VEL_LADD); every cell
+                                  without it must be WALL or MOVING: its slots hold the
+                                  populations bounced at it, not populations of its own */
 #define LBDEM_FLAG_NEAR 0x02   /* has a wall neighbour (set by lbdem_lbm_prepare_flags) */
 #define LBDEM_FLAG_WALL 0x04   /* bounce-back wall (CellType.OBSTACLE) */
 #define LBDEM_FLAG_MOVING 0x08 /* moving bounce-back wall (CellType.VEL_LADD), velocity in vel[] */
@@ -111,10 +113,14 @@ const char *lbdem_last_error(void);
  * wall-neighbour mask of the NEAR cells (bit q, 1..18: x + c_q is a wall; bit 31: some
  * neighbour is a moving wall; 0 elsewhere).  Call after every change of the flags.  It replaces
  * the per-direction cell-type tests of BasicLattice3D.stream (src/lbm3d/lbm_solver3d.py:216-236)
- * with one load per near-wall cell. */
+ * with one load per near-wall cell.  Populations bounced at a wall are kept in the wall cell's
+ * slots, so the layout of f depends on the flags: to change walls on a lattice that holds
+ * populations, read them with lbdem_lbm_read_f, change the flags and call this function, then
+ * store them back with lbdem_lbm_write_f (same t).  The next step then starts from the same
+ * post-stream state, as in the reference. */
 int lbdem_lbm_prepare_flags(const lbdem_lattice *lat, uint8_t *flags, uint32_t *nmask, void *stream);
 
-/* f <- feq(rho, vel) at active cells, natural layout (t = 0); lag state initialised.
+/* f <- feq(rho, vel) at active cells as the state t = 0; lag state initialised.
  * Replaces BasicLattice3D.initialize (src/lbm3d/lbm_solver3d.py:157-171), also
  * PSCLattice3D.initialize (src/lbdem/psclattice3d.py:60-78, lattice part),
  * EqIMBlattice3D.initialize (src/unlbdem/eqlattice.py:90-98),
@@ -146,7 +152,8 @@ int lbdem_lbm_macros(const lbdem_lattice *lat, const lbdem_step_args *a, float *
 int lbdem_lbm_read_f(const lbdem_lattice *lat, const lbdem_step_args *a, void *out,
                      int32_t out_f64, void *stream);
 
-/* Store unshifted natural-order populations (double if in_f64) as the state t = 0.
+/* Store unshifted natural-order populations (double if in_f64) at active cells as the
+ * post-stream state after a->t steps (the inverse of lbdem_lbm_read_f at the same t).
  * Backs `lattice.f.from_numpy()`. */
 int lbdem_lbm_write_f(const lbdem_lattice *lat, const lbdem_step_args *a, const void *in,
                       int32_t in_f64, void *stream);
@@ -331,10 +338,16 @@ typedef struct lbdem_diff_args {
 int lbdem_diff_init(const lbdem_diff_args *d, void *f, void *stream);
 /* f_out = S(C(f_in)) */
 int lbdem_diff_step(const lbdem_diff_args *d, const void *f_in, void *f_out, void *stream);
-/* lam_out = C'(f_n)^T S^T lam_in; g_eps[c] += dL/d eps_c; g_red[0] += dL/dtau,
- * g_red[1..3] += dL/dF (FP64) */
+/* lam_out = C'(f_n)^T S^T lam_in; g_eps[c] += dL/d eps_c; the FP64 contributions to
+ * dL/dtau, dL/dF (3) are added to g_red, an accumulator of lbdem_diff_red_doubles(d) doubles
+ * (four per workgroup, zeroed by the caller before the first step), whose total
+ * lbdem_diff_reduce returns. */
 int lbdem_diff_adjoint_step(const lbdem_diff_args *d, const void *f_n, const void *lam_in, void *lam_out,
                             void *g_eps, double *g_red, void *stream);
+int64_t lbdem_diff_red_doubles(const lbdem_diff_args *d);
+/* out[0] = dL/dtau, out[1..3] = dL/dF: the sum of the accumulator in a fixed order
+ * (deterministic, like lbdem_lbm_reduce) */
+int lbdem_diff_reduce(const lbdem_diff_args *d, const double *g_red, double *out, void *stream);
 /* u_x on the plane i = plane (ny*nz) */
 int lbdem_diff_outlet(const lbdem_diff_args *d, const void *f, void *u_plane, void *stream);
 /* lam = dL/df of the state f given dL/du_x on the plane (g_u, ny*nz); zero off the plane */

@@ -26,11 +26,31 @@ struct DiffArgs {
   void *lam_out;
   const void *eps;   // per-cell solid fraction (T)
   void *g_eps;       // per-cell dL/d eps accumulator (T)
-  double *g_red;     // 4 doubles: dL/dtau, dL/dF (3)
+  double *g_red;     // 4 doubles per workgroup: dL/dtau, dL/dF (3), accumulated over steps
   double tau, Fx, Fy, Fz;
   int plane;         // outlet plane index along x
   const void *g_u;   // dL/d u_x on the plane (ny*nz, T)
   void *u_plane;     // u_x on the plane (ny*nz, T)
+};
+
+// Periodic neighbour index x + c from three wrapped offsets per axis (computed once per cell;
+// each direction then costs two additions instead of three wraps and a multiply-add).
+struct NbrIdx {
+  int x[3], y[3], z[3];
+  __device__ __forceinline__ NbrIdx(const DiffArgs &a, int c) {
+    const int nyz = a.ny * a.nz;
+    const int i = c / nyz, j = (c - i * nyz) / a.nz, k = c - i * nyz - j * a.nz;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x[d] = wrapi(i + d - 1, a.nx) * nyz;
+      y[d] = wrapi(j + d - 1, a.ny) * a.nz;
+      z[d] = wrapi(k + d - 1, a.nz);
+    }
+  }
+  template <int cx, int cy, int cz>
+  __device__ __forceinline__ int at() const {
+    return x[cx + 1] + y[cy + 1] + z[cz + 1];
+  }
 };
 
 template <typename T>
@@ -84,13 +104,11 @@ __global__ __launch_bounds__(256) void k_diff_step(const DiffArgs a) {
   T rho, ux, uy, uz, B;
   psc_collide_fixed<T>(f, reinterpret_cast<const T *>(a.eps)[c], T(a.tau), T(a.Fx), T(a.Fy), T(a.Fz), g, rho, ux,
                        uy, uz, B);
-  const int nyz = a.ny * a.nz;
-  const int i = c / nyz, j = (c - i * nyz) / a.nz, k = c - i * nyz - j * a.nz;
+  const NbrIdx nb(a, c);
   const bool near = fl & LBDEM_FLAG_NEAR;  // a wall among the neighbours (bounce-back)
   static_for<0, Q>([&](auto qc) {
     constexpr int q = qc;
-    const int ii = wrapi(i + kCX[q], a.nx), jj = wrapi(j + kCY[q], a.ny), kk = wrapi(k + kCZ[q], a.nz);
-    const int m = (ii * a.ny + jj) * a.nz + kk;
+    const int m = nb.at<kCX[q], kCY[q], kCZ[q]>();
     if (q != 0 && near && (a.flags[m] & kMaskSolid)) G[kOPP[q] * N + c] = g[q];  // bounce back
     else G[q * N + m] = g[q];
   });
@@ -108,16 +126,14 @@ __global__ __launch_bounds__(256) void k_diff_adjoint(const DiffArgs a) {
     const T *F = reinterpret_cast<const T *>(a.f_in);
     const T *L = reinterpret_cast<const T *>(a.lam_in);
     T *Lo = reinterpret_cast<T *>(a.lam_out);
-    const int nyz = a.ny * a.nz;
-    const int i = c / nyz, j = (c - i * nyz) / a.nz, k = c - i * nyz - j * a.nz;
+    const NbrIdx nb(a, c);
     const bool near = fl & LBDEM_FLAG_NEAR;  // a wall among the neighbours (bounce-back)
     // mu = S^T lambda^{n+1}
     T mu[Q], f[Q];
     static_for<0, Q>([&](auto qc) {
       constexpr int q = qc;
       f[q] = F[q * N + c];
-      const int ii = wrapi(i + kCX[q], a.nx), jj = wrapi(j + kCY[q], a.ny), kk = wrapi(k + kCZ[q], a.nz);
-      const int m = (ii * a.ny + jj) * a.nz + kk;
+      const int m = nb.at<kCX[q], kCY[q], kCZ[q]>();
       if (q != 0 && near && (a.flags[m] & kMaskSolid)) mu[q] = L[kOPP[q] * N + c];
       else mu[q] = L[q * N + m];
     });
@@ -207,7 +223,9 @@ __global__ __launch_bounds__(256) void k_diff_adjoint(const DiffArgs a) {
     gfy = (double)dFy;
     gfz = (double)dFz;
   }
-  // block sums in a fixed order (one barrier), then one FP64 atomic per value and block
+  // block sums in a fixed order (one barrier), added to this workgroup's own accumulator slot:
+  // no atomics (same-address FP64 atomics from every workgroup cost a third of the step) and the
+  // final sum, lbdem_diff_reduce, runs in a fixed order: the gradients are reproducible
   __shared__ double sh[4][4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const double v0 = wave_sum(gt), v1 = wave_sum(gfx), v2 = wave_sum(gfy), v3 = wave_sum(gfz);
@@ -220,8 +238,27 @@ __global__ __launch_bounds__(256) void k_diff_adjoint(const DiffArgs a) {
   __syncthreads();
   if (threadIdx.x < 4) {
     const double r = (sh[threadIdx.x][0] + sh[threadIdx.x][1]) + (sh[threadIdx.x][2] + sh[threadIdx.x][3]);
-    unsafeAtomicAdd(a.g_red + threadIdx.x, r);
+    a.g_red[4 * (size_t)blockIdx.x + threadIdx.x] += r;
   }
+}
+
+// out[m] = sum over workgroups b of part[4 b + m], fixed order (one workgroup)
+__global__ __launch_bounds__(256) void k_diff_reduce(const double *part, int nblocks, double *out) {
+  __shared__ double sh[256][4];
+  double s[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) s[m] += part[4 * (size_t)b + m];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) sh[threadIdx.x][m] = s[m];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) sh[threadIdx.x][m] += sh[threadIdx.x + w][m];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = sh[0][threadIdx.x];
 }
 
 // u_x on the plane i = plane
@@ -363,6 +400,22 @@ int lbdem_diff_adjoint_step(const lbdem_diff_args *d, const void *f_n, const voi
   a.g_red = g_red;
   return launch_diff(LBDEM_DIFF_FN(k_diff_adjoint, d->precision), (unsigned)((a.ncell + 255) / 256), a, stream,
                      "lbdem_diff_adjoint_step");
+}
+
+int64_t lbdem_diff_red_doubles(const lbdem_diff_args *d) {
+  if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return 0;
+  return 4 * (((int64_t)d->nx * d->ny * d->nz + 255) / 256);
+}
+
+int lbdem_diff_reduce(const lbdem_diff_args *d, const double *g_red, double *out, void *stream) {
+  DiffArgs a;
+  int rc = fill_diff(d, a, "lbdem_diff_reduce");
+  if (rc) return rc;
+  LBDEM_REQUIRE(g_red && out, "lbdem_diff_reduce: null argument");
+  const int nblocks = (a.ncell + 255) / 256;
+  hipLaunchKernelGGL(k_diff_reduce, dim3(1), dim3(256), 0, as_stream(stream), g_red, nblocks, out);
+  LBDEM_CHECK_LAUNCH("lbdem_diff_reduce");
+  return LBDEM_OK;
 }
 
 int lbdem_diff_outlet(const lbdem_diff_args *d, const void *f, void *u_plane, void *stream) {

@@ -442,13 +442,17 @@ __device__ void resolve_new(const lbdem_dem_params &P, const lbdem_dem_state &S,
 
 // resolve_wall + evaluate_wall (demsolver.py:742-838) for particle i; forces added in place
 __device__ void resolve_walls(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
+  // the six activity flags are loaded together: one memory round trip instead of one per wall
+  unsigned wact = 0u;
+#pragma unroll
+  for (int w = 0; w < 6; ++w) wact |= (S.wcf_active[6 * i + w] != 0 ? 1u : 0u) << w;
   for (int w = 0; w < 6; ++w) {
     if (P.wall_type[w] == 1) continue;
     const double *nrm = P.wall_n[w];
     const double *x = S.pos + 3 * i;
     const double pd = x[0] * nrm[0] + x[1] * nrm[1] + x[2] * nrm[2] - P.wall_d[w];
     const int c = 6 * i + w;
-    if (S.wcf_active[c]) {
+    if ((wact >> w) & 1u) {
       if (-pd > S.radius[i]) {
         S.wcf_active[c] = 0;
         continue;
@@ -699,24 +703,62 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   // vl_off (lists unusable for this launch: no skin or a list overflowed), vl_over (build phase)
   __shared__ int vl_rebuild, vl_off, vl_over, vl_load;
   __shared__ double rmax_w[NT / 64];
+  static_assert(!RESIDENT || NT >= kResidentN, "the resident kernel runs one thread per particle");
+  // Verlet lists of the previous launch, loaded with the state (used only if still valid)
+  int32_t vl_cnt_in = 0, vl_nb_in[RESIDENT ? kNbr : 1], vl_valid_in = 0, vl_n_in = 0;
+  double vl_x_in[3], vl_skin_in = 0.0;
   if constexpr (RESIDENT) {
-    double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
-    for (int t = threadIdx.x; t < 3 * n; t += NT) {
+    // thread t < n copies particle t (n <= NT): every load is issued before the first LDS
+    // store, so the whole copy costs one memory round trip
+    const int t = threadIdx.x;
+    if (t < n) {
+      const double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
+      double v[7][3], q4[4], in9[9], fl[2][3];
 #pragma unroll
-      for (int a = 0; a < 7; ++a) r_vec[a][t] = gv[a][t];
-    }
-    for (int t = threadIdx.x; t < 4 * n; t += NT) r_quat[t] = G.quat[t];
-    for (int t = threadIdx.x; t < 9 * n; t += NT) r_inertia[t] = G.inertia[t];
-    for (int t = threadIdx.x; t < 3 * n; t += NT) {
-      r_fluid[0][t] = G.force_fluid[t];
-      r_fluid[1][t] = G.moment_fluid[t];
-    }
-    for (int t = threadIdx.x; t < n; t += NT) {
-      r_mass[t] = G.mass[t];
-      r_rad[t] = G.radius[t];
-      r_cfn[t] = G.cfn[t];
-      r_freeze[t] = G.freeze ? G.freeze[t] : 0;
-      r_fixvel[t] = G.fixvel ? G.fixvel[t] : 0;
+      for (int a = 0; a < 7; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[a][k] = gv[a][3 * t + k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q4[k] = G.quat[4 * t + k];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) in9[k] = G.inertia[9 * t + k];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        fl[0][k] = G.force_fluid[3 * t + k];
+        fl[1][k] = G.moment_fluid[3 * t + k];
+      }
+      const double mass = G.mass[t], rad = G.radius[t];
+      const int32_t cfn = G.cfn[t], frz = G.freeze ? G.freeze[t] : 0, fxv = G.fixvel ? G.fixvel[t] : 0;
+      if (vl_g) {
+        if (t == 0) {
+          vl_valid_in = vl_g[0];
+          vl_n_in = vl_g[1];
+          vl_skin_in = vl_xg[0];
+        }
+        vl_cnt_in = vl_g[2 + t];
+#pragma unroll
+        for (int m = 0; m < kNbr; ++m) vl_nb_in[m] = vl_g[2 + n + kNbr * t + m];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vl_x_in[k] = vl_xg[1 + 3 * t + k];
+      }
+#pragma unroll
+      for (int a = 0; a < 7; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r_vec[a][3 * t + k] = v[a][k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r_quat[4 * t + k] = q4[k];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) r_inertia[9 * t + k] = in9[k];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        r_fluid[0][3 * t + k] = fl[0][k];
+        r_fluid[1][3 * t + k] = fl[1][k];
+      }
+      r_mass[t] = mass;
+      r_rad[t] = rad;
+      r_cfn[t] = cfn;
+      r_freeze[t] = frz;
+      r_fixvel[t] = fxv;
     }
     S.pos = r_vec[0];
     S.vel = r_vec[1];
@@ -757,19 +799,21 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
       vl_off = vl_skin > 0.0 ? 0 : 1;
       vl_over = 0;
       // lists kept by the previous launch: valid flag, particle count and skin must match
-      vl_load = (!vl_off && vl_g && vl_g[0] == 1 && vl_g[1] == n && vl_xg[0] == vl_skin) ? 1 : 0;
+      vl_load = (!vl_off && vl_g && vl_valid_in == 1 && vl_n_in == n && vl_skin_in == vl_skin) ? 1 : 0;
       vl_rebuild = vl_load ? 0 : 1;
     }
     __syncthreads();
     if (vl_load) {
-      for (int t = threadIdx.x; t < n; t += NT) nb_cnt[t] = vl_g[2 + t];
-      for (int t = threadIdx.x; t < kNbr * n; t += NT) nb_list[t] = vl_g[2 + n + t];
-      for (int t = threadIdx.x; t < 3 * n; t += NT) x_ref[t] = vl_xg[1 + t];
-      __syncthreads();
-      // positions changed between launches (by the caller) by more than half the skin: rebuild
-      for (int i = threadIdx.x; i < n; i += NT) {
+      const int i = threadIdx.x;
+      if (i < n) {
+        nb_cnt[i] = vl_cnt_in;
+#pragma unroll
+        for (int m = 0; m < kNbr; ++m) nb_list[kNbr * i + m] = vl_nb_in[m];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) x_ref[3 * i + k] = vl_x_in[k];
+        // positions changed between launches (by the caller) by more than half the skin: rebuild
         double d[3];
-        periodic_delta(P, x_ref + 3 * i, posv + 3 * i, d);
+        periodic_delta(P, vl_x_in, posv + 3 * i, d);
         const double hs = 0.5 * vl_skin;
         if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > hs * hs) atomicOr(&vl_rebuild, 1);
       }
@@ -919,19 +963,25 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     __syncthreads();
   }
   if constexpr (RESIDENT) {
-    double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
-    for (int t = threadIdx.x; t < 3 * n; t += NT) {
+    const int t = threadIdx.x;
+    if (t < n) {  // particle t's state back to global memory (the caller's arrays)
+      double *gv[7] = {G.pos, G.vel, G.acc, G.force, G.omega, G.omega_dot, G.moment};
 #pragma unroll
-      for (int a = 0; a < 7; ++a) gv[a][t] = r_vec[a][t];
+      for (int a = 0; a < 7; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gv[a][3 * t + k] = r_vec[a][3 * t + k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) G.quat[4 * t + k] = r_quat[4 * t + k];
+      G.cfn[t] = r_cfn[t];
     }
-    for (int t = threadIdx.x; t < 4 * n; t += NT) G.quat[t] = r_quat[t];
-    for (int t = threadIdx.x; t < n; t += NT) G.cfn[t] = r_cfn[t];
     if (vl_g) {  // keep the Verlet lists for the next launch
       const bool keep = !vl_off && !vl_rebuild;
-      if (keep) {
-        for (int t = threadIdx.x; t < n; t += NT) vl_g[2 + t] = nb_cnt[t];
-        for (int t = threadIdx.x; t < kNbr * n; t += NT) vl_g[2 + n + t] = nb_list[t];
-        for (int t = threadIdx.x; t < 3 * n; t += NT) vl_xg[1 + t] = x_ref[t];
+      if (keep && t < n) {
+        vl_g[2 + t] = nb_cnt[t];
+#pragma unroll
+        for (int m = 0; m < kNbr; ++m) vl_g[2 + n + kNbr * t + m] = nb_list[kNbr * t + m];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vl_xg[1 + 3 * t + k] = x_ref[3 * t + k];
       }
       if (threadIdx.x == 0) {
         vl_g[0] = keep ? 1 : 0;

@@ -194,7 +194,7 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
   const int V = kern == 4 ? 4 : kern == 3 ? 2 : 1;
   const int tsize = lat->storage == LBDEM_STORE_F64 ? 8 : 4;
   const int block = ((lat->nz / V + 63) / 64) * 64;
-  const size_t lds = p.t_odd ? (size_t)10 * block * V * (tsize + 1) : 0;
+  const size_t lds = p.t_odd ? (size_t)10 * block * V * tsize : 0;
   const bool pencil_ok = (lat->nz % V == 0) && block <= 1024 && lds <= 65536;
   KernelFn kp = V == 4 ? t->pencil4[coll][lat->coupling][p.t_odd]
                 : V == 2 ? t->pencil2[coll][lat->coupling][p.t_odd] : t->pencil[coll][lat->coupling][p.t_odd];

@@ -9,6 +9,18 @@
 // walls; the wall directions come from the cell's neighbour mask) and wraps periodically at the
 // domain edge.  The row-tiled kernels wrap by construction and take the slow path only for NEAR
 // cells.
+// Bounce-back in the AA pattern, chosen so that every row-tiled store writes whole rows (a
+// store with some lanes masked off leaves partly written cache lines, which cost as much as the
+// rest of the near-wall handling together):
+//  - even step: a population f*_q leaving x towards a wall goes, like every other, into x's own
+//    slot OPP[q], with the Ladd term at moving walls; the odd gather reads it back from there.
+//    Wall cells store back the values they loaded, unchanged (they hold populations that the
+//    near-wall cells read in this step);
+//  - odd step: every cell, walls included, pushes all 19 populations; the one heading into a
+//    wall w = x + c_q lands in w's slot q (Ladd term applied by x), and the next even step of x
+//    reads direction OPP[q] from there.  A wall pushes back the values its target slots already
+//    hold (loaded in its gather): a near-wall target reads its bounced population from that slot
+//    in the same step.
 #pragma once
 #include <type_traits>
 #include <utility>
@@ -55,40 +67,72 @@ __device__ __forceinline__ bool interior(const KParams &p, int i, int j, int k) 
   return i > 0 && i < p.nx - 1 && j > 0 && j < p.ny - 1 && k > 0 && k < p.nz - 1;
 }
 
-// Post-stream (shifted) populations of cell c after t steps, t parity `odd`.
+// Post-stream (shifted) populations of cell c after t steps, t parity `odd`.  Direction q of a
+// NEAR cell whose upstream neighbour x - c_q is a wall is a bounced population: after an odd step
+// it sits in the wall's slot OPP[q], after an even step in the cell's own slot q.
 template <typename S>
 __device__ __forceinline__ void gather(const KParams &p, const S *__restrict__ F, int c, uint8_t fl,
                                        bool odd, typename Store<S>::T (&f)[Q]) {
   const size_t N = (size_t)p.ncell;
+  int i, j, k;
+  cell_ijk(p, c, i, j, k);
   if (!odd) {
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      f[q] = Store<S>::ld(F + q * N + c);
-    });
-  } else {
-    int i, j, k;
-    cell_ijk(p, c, i, j, k);
-    if (!(fl & LBDEM_FLAG_NEAR) && interior(p, i, j, k)) {
-      const int sx = p.ny * p.nz, sy = p.nz;
+    if (!(fl & LBDEM_FLAG_NEAR)) {
       static_for<0, Q>([&](auto qc) {
         constexpr int q = qc;
-        const int off = kCX[q] * sx + kCY[q] * sy + kCZ[q];
-        f[q] = Store<S>::ld(F + kOPP[q] * N + (c - off));
+        f[q] = Store<S>::ld(F + q * N + c);
       });
       return;
     }
-    const uint32_t nm = (fl & LBDEM_FLAG_NEAR) ? p.nmask[c] : 0u;
+    const uint32_t nm = p.nmask[c];
     static_for<0, Q>([&](auto qc) {
       constexpr int q = qc;
-      if constexpr (q == 0) {
-        f[0] = Store<S>::ld(F + c);
-      } else if ((nm >> kOPP[q]) & 1u) {
-        f[q] = Store<S>::ld(F + q * N + c);  // x - c_q is a wall: bounced back last step
-      } else {
+      if (q != 0 && ((nm >> kOPP[q]) & 1u))
         f[q] = Store<S>::ld(F + kOPP[q] * N + nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]));
-      }
+      else
+        f[q] = Store<S>::ld(F + q * N + c);
     });
+    return;
   }
+  if (!(fl & LBDEM_FLAG_NEAR) && interior(p, i, j, k)) {
+    const int sx = p.ny * p.nz, sy = p.nz;
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      const int off = kCX[q] * sx + kCY[q] * sy + kCZ[q];
+      f[q] = Store<S>::ld(F + kOPP[q] * N + (c - off));
+    });
+    return;
+  }
+  const uint32_t nm = (fl & LBDEM_FLAG_NEAR) ? p.nmask[c] : 0u;
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    if (q != 0 && ((nm >> kOPP[q]) & 1u))
+      f[q] = Store<S>::ld(F + q * N + c);
+    else
+      f[q] = Store<S>::ld(F + kOPP[q] * N + nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]));
+  });
+}
+
+// Stores the post-stream populations f of active cell c as the state after t steps (t parity
+// `odd`): the inverse of gather.  Every slot written here is gather's source of exactly one value.
+template <typename S>
+__device__ __forceinline__ void scatter_state(const KParams &p, S *__restrict__ F, int c, uint8_t fl, bool odd,
+                                              const typename Store<S>::T (&f)[Q]) {
+  const size_t N = (size_t)p.ncell;
+  int i, j, k;
+  cell_ijk(p, c, i, j, k);
+  const uint32_t nm = (fl & LBDEM_FLAG_NEAR) ? p.nmask[c] : 0u;
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    const bool bounced = q != 0 && ((nm >> kOPP[q]) & 1u);
+    if (!odd) {
+      if (bounced) Store<S>::st(F + kOPP[q] * N + nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]), f[q]);
+      else Store<S>::st(F + q * N + c, f[q]);
+    } else {
+      if (q == 0 || bounced) Store<S>::st(F + q * N + c, f[q]);
+      else Store<S>::st(F + kOPP[q] * N + nbr_index(p, i, j, k, -kCX[q], -kCY[q], -kCZ[q]), f[q]);
+    }
+  });
 }
 
 // 1/x in the collision: the hardware reciprocal (1 ulp) for FP32, a division for FP64
@@ -416,15 +460,10 @@ __global__ __launch_bounds__(256) void k_step(const KParams p) {
         Store<S>::st(F + c, f[0]);
       } else {
         const int m = nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q]);
-        if ((nm >> q) & 1u) {
-          T v = f[q];
-          if ((nm & kNbrMoving) && (p.flags[m] & LBDEM_FLAG_MOVING)) v -= ladd_term<q, T>(p, m, rho);
-          Store<S>::st(F + kOPP[q] * N + c, v);
-        } else if constexpr (!ODD) {
-          Store<S>::st(F + kOPP[q] * N + c, f[q]);
-        } else {
-          Store<S>::st(F + q * N + m, f[q]);
-        }
+        T v = f[q];
+        if (((nm >> q) & 1u) && (nm & kNbrMoving) && (p.flags[m] & LBDEM_FLAG_MOVING)) v -= ladd_term<q, T>(p, m, rho);
+        if constexpr (!ODD) Store<S>::st(F + kOPP[q] * N + c, v);
+        else Store<S>::st(F + q * N + m, v);  // into a wall cell too: read there by the next even step
       }
     });
   }
@@ -432,13 +471,14 @@ __global__ __launch_bounds__(256) void k_step(const KParams p) {
 
 using StepKernel = void (*)(const KParams);
 
-// f <- feq(rho, u) at active cells, natural layout; lag state initialised
+// f <- feq(rho, u) at active cells as the state t = 0; lag state initialised
 template <typename S, int COUP>
 __global__ __launch_bounds__(256) void k_init(const KParams p) {
   using T = typename Store<S>::T;
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= p.ncell) return;
-  if (!(p.flags[c] & LBDEM_FLAG_ACTIVE)) return;
+  const uint8_t fl = p.flags[c];
+  if (!(fl & LBDEM_FLAG_ACTIVE)) return;
   S *__restrict__ F = reinterpret_cast<S *>(p.f);
   const size_t N = (size_t)p.ncell;
   const T rho = (T)p.rho_out[c];
@@ -447,11 +487,13 @@ __global__ __launch_bounds__(256) void k_init(const KParams p) {
   const T usq = ux * ux + uy * uy + uz * uz;
   T fvf = T(1);
   if (COUP == LBDEM_COUPLE_VANS && p.volfrac) fvf = T(1) - (T)p.volfrac[c];
+  T fe[Q];
   static_for<0, Q>([&](auto qc) {
     constexpr int q = qc;
     const T cu = cdot<q>(ux, uy, uz);
-    Store<S>::st(F + q * N + c, T(kW[q]) * (dr + rho * fvf * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq)));
+    fe[q] = T(kW[q]) * (dr + rho * fvf * (T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq));
   });
+  scatter_state<S>(p, F, c, fl, false, fe);
   if (p.lag) {
     T *lag = reinterpret_cast<T *>(p.lag);
     lag[c] = rho;
@@ -519,17 +561,21 @@ __global__ __launch_bounds__(256) void k_read_f(const KParams p) {
 
 template <typename S>
 __global__ __launch_bounds__(256) void k_write_f(const KParams p) {
+  using T = typename Store<S>::T;
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= p.ncell) return;
-  if (!(p.flags[c] & LBDEM_FLAG_ACTIVE)) return;
+  const uint8_t fl = p.flags[c];
+  if (!(fl & LBDEM_FLAG_ACTIVE)) return;
   const size_t N = (size_t)p.ncell;
   S *F = reinterpret_cast<S *>(p.f);
+  T f[Q];
   static_for<0, Q>([&](auto qc) {
     constexpr int q = qc;
     const double v = p.out_f64 ? reinterpret_cast<const double *>(p.in)[q * N + c]
                                : (double)reinterpret_cast<const float *>(p.in)[q * N + c];
-    Store<S>::st(F + q * N + c, (typename Store<S>::T)(v - kW[q] * p.rho0));
+    f[q] = (T)(v - kW[q] * p.rho0);
   });
+  scatter_state<S>(p, F, c, fl, p.t_odd != 0, f);
 }
 
 // per-workgroup FP64 partial sums: rho, rho u (3), rho |u|^2 / 2
@@ -637,7 +683,6 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
   T *buf = reinterpret_cast<T *>(smem);  // [kNZ][ncz]
   const int nz = p.nz;
   const int ncz = blockDim.x * V;         // row cells covered by the block (>= nz)
-  unsigned char *okb = smem + sizeof(T) * kNZ * ncz;  // [kNZ][ncz] push validity
   const int row = blockIdx.x;
   const int i = row / p.ny, j = row - (row / p.ny) * p.ny;
   const int k0 = threadIdx.x * V;
@@ -670,6 +715,13 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
         constexpr int q = qc;
         T t[V];
         Vec<S, V>::ld(F + (q * N + (size_t)base) + uk0, t);
+        if (q != 0 && ((nm_any >> kOPP[q]) & 1u)) {  // bounced at the wall x - c_q: in the wall's slot
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if ((nmv[v] >> kOPP[q]) & 1u)
+              t[v] = Store<S>::ld(F + kOPP[q] * N + nbr_index(p, i, j, k0 + v, -kCX[q], -kCY[q], -kCZ[q]));
+          }
+        }
 #pragma unroll
         for (int v = 0; v < V; ++v) f[v][q] = t[v];
       });
@@ -707,12 +759,19 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
       });
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        if (fl[v] & LBDEM_FLAG_NEAR) {  // populations bounced back at a wall last step
-          const int k = k0 + v;
+        const int k = k0 + v;
+        if (fl[v] & LBDEM_FLAG_NEAR) {  // bounced at the wall x - c_q last step: in the own slot q
           const uint32_t nm = nmv[v];
           static_for<1, Q>([&](auto qc) {
             constexpr int q = qc;
             if ((nm >> kOPP[q]) & 1u) f[v][q] = Store<S>::ld(F + q * N + base + k);
+          });
+        } else if (!act[v]) {
+          // A wall pushes back what its targets hold: a target x + c_q that is a fluid cell
+          // reads that slot (its bounced population) in this step, and no other thread writes it.
+          static_for<0, Q>([&](auto qc) {
+            constexpr int q = qc;
+            f[v][q] = Store<S>::ld(F + q * N + nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q]));
           });
         }
       }
@@ -727,24 +786,28 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
     if (act[v]) rho[v] = collide<COLL, COUP, T>(p, base + k0 + v, i, j, k0 + v, f[v]);
   }
 
-  // ---------------------------------------------------------------- scatter
+  // Ladd term of the populations heading into a moving wall (wall neighbours are rare: one
+  // branch per direction on the union of this thread's masks)
+  auto moving_wall = [&](auto qc, int v, T &val) {
+    constexpr int q = decltype(qc)::value;
+    if (((nmv[v] >> q) & 1u) && (nmv[v] & kNbrMoving)) {
+      const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
+      if (p.flags[m] & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
+    }
+  };
+
+  // ---------------------------------------------------------------- scatter (whole rows)
   if constexpr (!ODD) {
     if (lane_in) {
       static_for<0, Q>([&](auto qc) {
         constexpr int q = qc;
         T t[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) t[v] = f[v][q];
-        if (q != 0 && (nm_any & kNbrMoving)) {  // moving-wall neighbours are rare: one branch
+        for (int v = 0; v < V; ++v) t[v] = act[v] ? f[v][q] : f[v][kOPP[q]];  // walls: unchanged
+        if (q != 0 && (nm_any & kNbrMoving)) {
 #pragma unroll
-          for (int v = 0; v < V; ++v) {
-            if (((nmv[v] >> q) & 1u) && (nmv[v] & kNbrMoving)) {
-              const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
-              if (p.flags[m] & LBDEM_FLAG_MOVING) t[v] -= ladd_term<q, T>(p, m, rho[v]);
-            }
-          }
+          for (int v = 0; v < V; ++v) moving_wall(qc, v, t[v]);
         }
-        // inactive cells' slots are never read, so they may receive garbage here
         Vec<S, V>::st(F + (kOPP[q] * N + (size_t)base) + uk0, t);
       });
     }
@@ -755,44 +818,18 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
       constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
       const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
       if (lane_in) {
-        bool ok[V];
+        T t[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) ok[v] = act[v];
-        if (q != 0 && ((nm_any >> q) & 1u)) {  // a wall in direction q: one branch per direction
+        for (int v = 0; v < V; ++v) t[v] = f[v][q];
+        if (q != 0 && (nm_any & kNbrMoving)) {
 #pragma unroll
-          for (int v = 0; v < V; ++v) {
-            if (act[v] && ((nmv[v] >> q) & 1u)) {  // bounce back into the own opposite slot
-              T val = f[v][q];
-              if (nmv[v] & kNbrMoving) {
-                const int m = nbr_index(p, i, j, k0 + v, cx, cy, cz);
-                if (p.flags[m] & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
-              }
-              Store<S>::st(F + kOPP[q] * N + base + k0 + v, val);
-              ok[v] = false;
-            }
-          }
+          for (int v = 0; v < V; ++v) moving_wall(qc, v, t[v]);
         }
         if constexpr (cz == 0) {
-          S *dst = F + (q * N + (size_t)trow * nz) + uk0;
-          bool all = true;
-#pragma unroll
-          for (int v = 0; v < V; ++v) all = all && ok[v];
-          if (all) {
-            T t[V];
-#pragma unroll
-            for (int v = 0; v < V; ++v) t[v] = f[v][q];
-            Vec<S, V>::st(dst, t);
-          } else {
-#pragma unroll
-            for (int v = 0; v < V; ++v)
-              if (ok[v]) Store<S>::st(dst + v, f[v][q]);
-          }
+          Vec<S, V>::st(F + (q * N + (size_t)trow * nz) + uk0, t);
         } else {
 #pragma unroll
-          for (int v = 0; v < V; ++v) {
-            buf[zslot<q>() * ncz + k0 + v] = f[v][q];
-            okb[zslot<q>() * ncz + k0 + v] = ok[v] ? 1 : 0;
-          }
+          for (int v = 0; v < V; ++v) buf[zslot<q>() * ncz + k0 + v] = t[v];
         }
       }
     });
@@ -803,24 +840,14 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
         constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
         if constexpr (cz != 0) {
           const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
-          S *dst = F + (q * N + (size_t)trow * nz) + uk0;
           T t[V];
-          bool ok[V], all = true;
 #pragma unroll
           for (int v = 0; v < V; ++v) {
             int ks = k0 + v - cz;  // the cell pushing into (row trow, k0 + v)
             ks = ks < 0 ? ks + nz : (ks >= nz ? ks - nz : ks);
             t[v] = buf[zslot<q>() * ncz + ks];
-            ok[v] = okb[zslot<q>() * ncz + ks] != 0;
-            all = all && ok[v];
           }
-          if (all) {
-            Vec<S, V>::st(dst, t);
-          } else {
-#pragma unroll
-            for (int v = 0; v < V; ++v)
-              if (ok[v]) Store<S>::st(dst + v, t[v]);
-          }
+          Vec<S, V>::st(F + (q * N + (size_t)trow * nz) + uk0, t);
         }
       });
     }

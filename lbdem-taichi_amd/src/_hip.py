@@ -27,7 +27,7 @@ class Lattice(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class StepArgs(ctypes.Structure):

@@ -31,6 +31,8 @@ _hip.register({
     "lbdem_diff_adjoint_step": (ctypes.c_int, [_P(DiffArgs), _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbdem_diff_outlet": (ctypes.c_int, [_P(DiffArgs), _vp, _vp, _vp]),
     "lbdem_diff_seed": (ctypes.c_int, [_P(DiffArgs), _vp, _vp, _vp, _vp]),
+    "lbdem_diff_red_doubles": (ctypes.c_int64, [_P(DiffArgs)]),
+    "lbdem_diff_reduce": (ctypes.c_int, [_P(DiffArgs), _vp, _vp, _vp]),
 })
 
 
@@ -112,7 +114,7 @@ class DiffPSCChannel:
         lam, lam2 = self._state(), self._state()
         _hip.check(L.lbdem_diff_seed(a, _hip.ptr(self._fT), _hip.ptr(g_u), _hip.ptr(lam), s), "lbdem_diff_seed")
         g_eps = torch.zeros(self.N, dtype=self.dtype, device=self.device)
-        g_red = torch.zeros(4, dtype=torch.float64, device=self.device)
+        g_red = torch.zeros(L.lbdem_diff_red_doubles(a), dtype=torch.float64, device=self.device)
         T, K = self._steps, self.K
         buf = torch.empty((K, 19, self.N), dtype=self.dtype, device=self.device)
         for seg in range((T + K - 1) // K - 1, -1, -1):
@@ -126,8 +128,10 @@ class DiffPSCChannel:
                                                      _hip.ptr(g_eps), _hip.ptr(g_red), s),
                            "lbdem_diff_adjoint_step")
                 lam, lam2 = lam2, lam
-        gF = g_red[1:4].clone()
-        gtau = g_red[0].clone()
+        tot = torch.empty(4, dtype=torch.float64, device=self.device)
+        _hip.check(L.lbdem_diff_reduce(a, _hip.ptr(g_red), _hip.ptr(tot), s), "lbdem_diff_reduce")
+        gF = tot[1:4].clone()
+        gtau = tot[0].clone()
         return gF, gtau, g_eps.view(self.nx, self.ny, self.nz)
 
 

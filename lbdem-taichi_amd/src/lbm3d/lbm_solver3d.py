@@ -49,9 +49,7 @@ class _PopulationField:
             raise ValueError(f"expected shape {(L.Nx, L.Ny, L.Nz, 19)}, got {arr.shape}")
         L._prepare()
         src = torch.from_numpy(np.ascontiguousarray(arr.reshape(L.N, 19).T)).to(L.device)
-        _hip.check(_hip.lib().lbdem_lbm_write_f(L._lat_c, L._args_c(), _hip.ptr(src), 1, _hip.stream()),
-                   "lbdem_lbm_write_f")
-        L.t = 0
+        L._write_populations(src)
         L._macro_t = -1
 
     def __getitem__(self, idx):
@@ -186,10 +184,27 @@ class BasicLattice3D:
         self._ct_version = self.CT.version
         self._active_np = active
 
+    def _write_populations(self, src):
+        """Store unshifted FP64 populations (19, N) as the state after self.t steps."""
+        _hip.check(_hip.lib().lbdem_lbm_write_f(self._lat_c, self._args_c(), _hip.ptr(src), 1, _hip.stream()),
+                   "lbdem_lbm_write_f")
+        self._populated = True
+
     def _prepare(self):
         """Bring device inputs up to date before a kernel launch."""
         if self.CT.version != self._ct_version:
+            # The populations' layout depends on the walls (bounced populations are kept in the
+            # wall cells): read them out with the old flags and store them back with the new ones,
+            # so the next step starts from the same post-stream state, as in the reference.
+            snap = None
+            if getattr(self, "_populated", False):
+                snap = torch.empty((19, self.N), dtype=torch.float64, device=self.device)
+                _hip.check(_hip.lib().lbdem_lbm_read_f(self._lat_c, self._args_c(), _hip.ptr(snap), 1,
+                                                       _hip.stream()), "lbdem_lbm_read_f")
             self._build_flags()
+            if snap is not None:
+                self._sync_lattice_struct()
+                self._write_populations(snap)
         self.CT.sync()
         self.vel.sync()
         self.rho.sync()
@@ -215,6 +230,7 @@ class BasicLattice3D:
         self.t = 0
         _hip.check(_hip.lib().lbdem_lbm_init(self._lat_c, self._args_c(), _hip.ptr(self.rho.dev), _hip.stream()),
                    "lbdem_lbm_init")
+        self._populated = True
         self._macro_t = 0
         if self._lag is not None:
             self.omega.touch()

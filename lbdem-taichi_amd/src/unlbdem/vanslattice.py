@@ -119,4 +119,5 @@ class Unresolvedlattice3D(BasicLattice3D):
         self.t = 0
         _hip.check(_hip.lib().lbdem_lbm_init(self._lat_c, ctypes.byref(a), _hip.ptr(self.rho.dev),
                                              _hip.stream()), "lbdem_lbm_init")
+        self._populated = True
         self._macro_t = 0
