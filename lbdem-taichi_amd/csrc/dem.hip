@@ -397,6 +397,19 @@ __device__ void eval_pair(const lbdem_dem_params &P, const lbdem_dem_state &S, i
   }
 }
 
+// gap < 0 exactly as resolve_existing computes it (|d| - r_i - r_j < 0), with the square root
+// skipped for pairs whose squared distance exceeds (r_i + r_j)^2 (1 + 1e-9): the computed gap of
+// those is at least 5e-10 (r_i + r_j), far above its rounding error, so the answer is the same.
+__device__ __forceinline__ bool touching(const lbdem_dem_params &P, const double *xi, double ri, const double *xj,
+                                         double rj) {
+  double d[3];
+  periodic_delta(P, xi, xj, d);
+  const double x2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const double a = ri + rj;
+  if (x2 > a * a * (1.0 + 1e-9)) return false;
+  return sqrt(x2) - ri - rj < 0;
+}
+
 // Phase 1 of resolve (demsolver.py:638-668): pairs with an active history slot are evaluated
 // (gap < 0) or released (gap >= 0).  Returns 1 when the pair needs a new slot.  Only reads
 // slots of other pairs, so it can run for all pairs concurrently.
@@ -440,26 +453,51 @@ __device__ void resolve_new(const lbdem_dem_params &P, const lbdem_dem_state &S,
   eval_pair(P, S, i, j, off, d, dist, dist - S.radius[i] - S.radius[j]);
 }
 
-// resolve_wall + evaluate_wall (demsolver.py:742-838) for particle i; forces added in place
-__device__ void resolve_walls(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
-  // the six activity flags are loaded together: one memory round trip instead of one per wall
-  unsigned wact = 0u;
+// particle i's six wall-contact flags (wcf_active) as a bit mask, and back (changed bits only)
+__device__ __forceinline__ unsigned load_wall_mask(const lbdem_dem_state &S, int i) {
+  unsigned m = 0u;
 #pragma unroll
-  for (int w = 0; w < 6; ++w) wact |= (S.wcf_active[6 * i + w] != 0 ? 1u : 0u) << w;
+  for (int w = 0; w < 6; ++w) m |= (S.wcf_active[6 * i + w] != 0 ? 1u : 0u) << w;
+  return m;
+}
+__device__ __forceinline__ void store_wall_mask(const lbdem_dem_state &S, int i, unsigned before, unsigned after) {
+#pragma unroll
+  for (int w = 0; w < 6; ++w)
+    if (((before ^ after) >> w) & 1u) S.wcf_active[6 * i + w] = (int32_t)((after >> w) & 1u);
+}
+
+// resolve_wall + evaluate_wall (demsolver.py:742-838) for particle i; forces added in place.
+// wact: bit w = contact with wall w active (wcf_active), updated here; the caller keeps it.
+__device__ void resolve_walls(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, unsigned &wact) {
+  // walls within reach, all six tested without branches; the loop below visits only those and
+  // the walls with an active contact (usually none)
+  unsigned reach = 0u;
+  {
+    const double *x = S.pos + 3 * i;
+    const double ri = S.radius[i];
+#pragma unroll
+    for (int w = 0; w < 6; ++w) {
+      const double *nrm = P.wall_n[w];
+      const double pd = x[0] * nrm[0] + x[1] * nrm[1] + x[2] * nrm[2] - P.wall_d[w];
+      reach |= (P.wall_type[w] != 1 && -pd <= ri) ? 1u << w : 0u;
+    }
+  }
+  const unsigned visit = reach | wact;
+  if (!visit) return;
   for (int w = 0; w < 6; ++w) {
-    if (P.wall_type[w] == 1) continue;
+    if (!((visit >> w) & 1u) || P.wall_type[w] == 1) continue;
     const double *nrm = P.wall_n[w];
     const double *x = S.pos + 3 * i;
     const double pd = x[0] * nrm[0] + x[1] * nrm[1] + x[2] * nrm[2] - P.wall_d[w];
     const int c = 6 * i + w;
     if ((wact >> w) & 1u) {
       if (-pd > S.radius[i]) {
-        S.wcf_active[c] = 0;
+        wact &= ~(1u << w);
         continue;
       }
     } else {
       if (!(-pd <= S.radius[i])) continue;
-      S.wcf_active[c] = 1;
+      wact |= 1u << w;
 #pragma unroll
       for (int k = 0; k < 3; ++k) S.wcf_shear[3 * c + k] = 0.0;
     }
@@ -526,18 +564,29 @@ __device__ __forceinline__ void wrap_position(const lbdem_dem_params &P, double 
 
 __device__ void rotate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
   const double dt = P.dt;
-  double R[9], Ri[9], Ii[9], Ml[3], wl[3], Iw[3], cr[3], t[3], wdl[3], alpha[3];
-  quat_rot(S.quat + 4 * i, R);
-  matvec(R, S.moment + 3 * i, Ml);
-  matvec(R, S.omega + 3 * i, wl);
-  matvec(S.inertia + 9 * i, wl, Iw);
-  cross3(wl, Iw, cr);
+  double alpha[3];
+  const double *In = S.inertia + 9 * i;
+  if (In[1] == 0.0 && In[2] == 0.0 && In[3] == 0.0 && In[5] == 0.0 && In[6] == 0.0 && In[7] == 0.0 &&
+      In[0] == In[4] && In[0] == In[8]) {
+    // isotropic inertia c I (spheres): R^-1 (cI)^-1 (R M - w_l x c w_l) = M / c; equal to the
+    // general expression up to its rounding
+    const double ic = 1.0 / In[0];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) t[k] = Ml[k] - cr[k];
-  inv3(S.inertia + 9 * i, Ii);
-  matvec(Ii, t, wdl);
-  inv3(R, Ri);
-  matvec(Ri, wdl, alpha);
+    for (int k = 0; k < 3; ++k) alpha[k] = S.moment[3 * i + k] * ic;
+  } else {
+    double R[9], Ri[9], Ii[9], Ml[3], wl[3], Iw[3], cr[3], t[3], wdl[3];
+    quat_rot(S.quat + 4 * i, R);
+    matvec(R, S.moment + 3 * i, Ml);
+    matvec(R, S.omega + 3 * i, wl);
+    matvec(In, wl, Iw);
+    cross3(wl, Iw, cr);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] = Ml[k] - cr[k];
+    inv3(In, Ii);
+    matvec(Ii, t, wdl);
+    inv3(R, Ri);
+    matvec(Ri, wdl, alpha);
+  }
   double *w = S.omega + 3 * i, *q = S.quat + 4 * i;
 #pragma unroll
   for (int k = 0; k < 3; ++k) w[k] += 0.5 * (S.omega_dot[3 * i + k] + alpha[k]) * dt;
@@ -557,8 +606,8 @@ __device__ void rotate(const lbdem_dem_params &P, const lbdem_dem_state &S, int 
 }
 
 // walls, apply_body_force (demsolver.py:437-448) and update (:514-636) for particle i
-__device__ void walls_and_integrate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
-  resolve_walls(P, S, i);
+__device__ void walls_and_integrate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, unsigned &wact) {
+  resolve_walls(P, S, i, wact);
   const double dt = P.dt;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -637,7 +686,10 @@ __global__ __launch_bounds__(256) void k_dem_integrate(const lbdem_dem_params P,
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i == 0 && *npairs > (unsigned long long)cap) atomicMax(S.status + 4, (int)min(*npairs, 0x7fffffffull));
   if (i >= P.n) return;
-  walls_and_integrate(P, S, i);
+  const unsigned w0 = load_wall_mask(S, i);
+  unsigned w = w0;
+  walls_and_integrate(P, S, i, w);
+  store_wall_mask(S, i, w0, w);
 }
 __global__ __launch_bounds__(256) void k_dem_compact(const lbdem_dem_params P, const lbdem_dem_state S,
                                                      unsigned long long *npairs) {
@@ -695,6 +747,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   __shared__ double r_inertia[9 * kRes];
   __shared__ double r_fluid[2][3 * kRes];
   __shared__ int32_t r_freeze[kRes], r_fixvel[kRes];
+  __shared__ uint32_t r_wact[kRes];  // wall-contact bits (resident kernel)
   __shared__ int32_t nb_cnt[kRes], nb_list[kNbr * kRes];  // Verlet lists (resident kernel)
   __shared__ double x_ref[3 * kRes];                       // positions at the last list build
   __shared__ double vl_skin;
@@ -705,6 +758,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
   __shared__ double rmax_w[NT / 64];
   static_assert(!RESIDENT || NT >= kResidentN, "the resident kernel runs one thread per particle");
   // Verlet lists of the previous launch, loaded with the state (used only if still valid)
+  unsigned wact_in = 0u;
   int32_t vl_cnt_in = 0, vl_nb_in[RESIDENT ? kNbr : 1], vl_valid_in = 0, vl_n_in = 0;
   double vl_x_in[3], vl_skin_in = 0.0;
   if constexpr (RESIDENT) {
@@ -727,6 +781,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
         fl[0][k] = G.force_fluid[3 * t + k];
         fl[1][k] = G.moment_fluid[3 * t + k];
       }
+      wact_in = load_wall_mask(G, t);
       const double mass = G.mass[t], rad = G.radius[t];
       const int32_t cfn = G.cfn[t], frz = G.freeze ? G.freeze[t] : 0, fxv = G.fixvel ? G.fixvel[t] : 0;
       if (vl_g) {
@@ -754,6 +809,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
         r_fluid[0][3 * t + k] = fl[0][k];
         r_fluid[1][3 * t + k] = fl[1][k];
       }
+      r_wact[t] = wact_in;
       r_mass[t] = mass;
       r_rad[t] = rad;
       r_cfn[t] = cfn;
@@ -855,11 +911,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
           if (h[k] < 0) continue;
           for (int j = head[h[k] & (kSmallSlots - 1)]; j >= 0; j = next[j]) {
             if (j <= i || chash[j] != h[k]) continue;
-            if (!has_slots) {  // no history slot to update: only a touching pair needs work
-              double d[3];
-              periodic_delta(P, xi, posv + 3 * j, d);
-              if (!(norm3(d) - ri - radv[j] < 0)) continue;
-            }
+            if (!has_slots && !touching(P, xi, ri, posv + 3 * j, radv[j])) continue;  // nothing to update
             if (resolve_existing(P, S, i, j)) {
               const int slot = atomicAdd(&nnew, 1);
               if (slot < newcap) {
@@ -873,11 +925,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     };
     // one pair (i, j > i) of a candidate list: touching pairs and pairs with history are resolved
     auto visit = [&](int i, const double *xi, double ri, bool has_slots, int j) {
-      if (!has_slots) {  // no history slot to update: only a touching pair needs work
-        double d[3];
-        periodic_delta(P, xi, posv + 3 * j, d);
-        if (!(norm3(d) - ri - radv[j] < 0)) return;
-      }
+      if (!has_slots && !touching(P, xi, ri, posv + 3 * j, radv[j])) return;  // nothing to update
       if (resolve_existing(P, S, i, j)) {
         const int slot = atomicAdd(&nnew, 1);
         if (slot < newcap) {
@@ -949,7 +997,14 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
     __syncthreads();
     // integration and slot compaction touch particle i's own data only
     for (int i = threadIdx.x; i < n; i += NT) {
-      walls_and_integrate(P, S, i);
+      if constexpr (RESIDENT) {
+        walls_and_integrate(P, S, i, r_wact[i]);
+      } else {
+        const unsigned w0 = load_wall_mask(S, i);
+        unsigned w = w0;
+        walls_and_integrate(P, S, i, w);
+        store_wall_mask(S, i, w0, w);
+      }
       compact_slots(P, S, i);
       if constexpr (RESIDENT) {
         if (!vl_off) {  // moved more than half the skin since the build: rebuild next substep
@@ -973,6 +1028,7 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
 #pragma unroll
       for (int k = 0; k < 4; ++k) G.quat[4 * t + k] = r_quat[4 * t + k];
       G.cfn[t] = r_cfn[t];
+      store_wall_mask(G, t, wact_in, r_wact[t]);
     }
     if (vl_g) {  // keep the Verlet lists for the next launch
       const bool keep = !vl_off && !vl_rebuild;

@@ -48,6 +48,7 @@ __device__ __forceinline__ void grain_box(const PscArgs &a, int g, double &xc, d
 // Each grain's box is shared by kSplit workgroups (blockIdx.y), so that a few hundred grains
 // still fill the chip.
 constexpr int kSplit = 8;
+constexpr int kMapChunk = 1024;  // cells classified per workgroup between two counting passes
 
 // min { x >= 0 : sqrt(x) >= r } for the correctly rounded sqrt: sqrt(x) < r <=> x < bound,
 // which replaces one FP64 square root per sub-cell sample with a compare.
@@ -98,19 +99,34 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
     }
   }
   const int ni = b[1] - b[0], nj = b[3] - b[2], nk = b[5] - b[4];
-  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;  // uniform over the workgroup
   const int tot = ni * nj * nk;
   const uint32_t gid = (uint32_t)(kMaxGrain - g);
-  for (int t = blockIdx.y * 256 + threadIdx.x; t < tot; t += 256 * kSplit) {
-    const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
-    const int c = (i * a.ny + j) * a.nz + k;
-    if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) continue;
-    const double dist = sqrt((xc - i) * (xc - i) + (yc - j) * (yc - j) + (zc - k) * (zc - k));
-    if (dist >= r + half_diag) continue;
-    uint32_t key;
-    if (dist <= r - half_diag) {
-      key = kFullBit | gid;
-    } else {
+  // Cells cut by the sphere surface need 125 sub-cell samples, the others one distance: in
+  // chunks of kMapChunk cells, full and empty cells are settled at once and the cut cells are
+  // queued in LDS, then counted one per lane with every lane busy (evaluated where they fall,
+  // about three lanes in four would idle through each count).
+  __shared__ int queue[kMapChunk];
+  __shared__ int qn;
+  for (int m0 = 0; (m0 * kSplit + (int)blockIdx.y) * 256 < tot; m0 += kMapChunk / 256) {
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+    for (int mm = 0; mm < kMapChunk / 256; ++mm) {
+      const int t = ((m0 + mm) * kSplit + (int)blockIdx.y) * 256 + (int)threadIdx.x;
+      if (t >= tot) break;
+      const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
+      const int c = (i * a.ny + j) * a.nz + k;
+      if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) continue;
+      const double dist = sqrt((xc - i) * (xc - i) + (yc - j) * (yc - j) + (zc - k) * (zc - k));
+      if (dist >= r + half_diag) continue;
+      if (dist <= r - half_diag) atomicMax(a.key + c, kFullBit | gid);
+      else queue[atomicAdd(&qn, 1)] = t;
+    }
+    __syncthreads();
+    const int nq = qn;
+    for (int e = threadIdx.x; e < nq; e += 256) {
+      const int t = queue[e];
+      const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
       int cnt = 0;
       for (int ii = 0; ii < 5; ++ii) {
         const double px = (double)i - 0.4 + 0.2 * ii;
@@ -124,10 +140,9 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
           }
         }
       }
-      if (cnt == 0) continue;
-      key = ((uint32_t)cnt << 23) | gid;
+      if (cnt > 0) atomicMax(a.key + (i * a.ny + j) * a.nz + k, ((uint32_t)cnt << 23) | gid);
     }
-    atomicMax(a.key + c, key);
+    __syncthreads();  // the queue is refilled by the next chunk
   }
 }
 

@@ -89,6 +89,37 @@ def test_dem_trajectories_match_oracle(model, path):
     assert gpu.status()["slot_overflow"] == 0
 
 
+def test_dem_anisotropic_inertia_matches_oracle():
+    """Full inertia tensors (principal moments 1 : 1.4 : 0.7, randomly rotated): the general
+    rotation update (body frame, inverse inertia, Euler term), where spheres take the isotropic
+    shortcut M / I."""
+    rng = np.random.default_rng(21)
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    cfg = make_config(model="hertz", box=box, dt=5e-5, kn=2e3, kt=2e3, zeta=0.3)
+    rad, mass, pos, vel = random_packing(120, 0.045, box, rng, v=0.5, polydisperse=0.3)
+    gpu = gpu_solver(cfg, rad, mass, pos, vel, path=1)
+    orc = oracle_solver(cfg, rad, mass, pos, vel)
+    n = len(rad)
+    inertia = np.zeros((n, 3, 3))
+    for i in range(n):
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        c = 0.4 * mass[i] * rad[i] ** 2
+        inertia[i] = q @ np.diag([c, 1.4 * c, 0.7 * c]) @ q.T
+    orc["inertia"][...] = inertia
+    gpu.gf.inertia.from_numpy(inertia.reshape(n, 9))
+    omega0 = rng.uniform(-20.0, 20.0, (n, 3))
+    orc["omega"][...] = omega0
+    gpu.gf.omega.from_numpy(omega0)
+    for _ in range(8):
+        gpu.run_substeps(50)
+        orc.step(50)
+    torch.cuda.synchronize()
+    om = gpu.gf.omega.to_numpy()
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+    assert np.abs(om - orc["omega"]).max() < 1e-6 * max(1.0, np.abs(orc["omega"]).max())
+    assert np.abs(gpu.gf.quaternion.to_numpy() - orc["quat"]).max() < 1e-9
+
+
 def test_dem_periodic_axis():
     gpu, orc = _run_pair("hertz", 0, periodic=(True, False, False), seed=4)
     assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
