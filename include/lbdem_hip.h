@@ -199,9 +199,12 @@ int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_gr
 
 /* Per-grain hydrodynamic force and torque (physical units, FP64, fixed-order reduction) from
  * the per-cell force the PSC collision wrote.  Replaces PSCLattice3D.lattice2grains
- * (src/lbdem/psclattice3d.py:316-358). */
+ * (src/lbdem/psclattice3d.py:316-358).  stale_key / stale_bbox (may be NULL): a second key
+ * array holding an earlier mapping with its boxes; the same launch zeroes those keys, so the
+ * next lbdem_psc_map into that array needs no clear pass (two key arrays used in turn). */
 int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u,
-                     const uint32_t *key, const float *cell_force, void *stream);
+                     const uint32_t *key, const float *cell_force, uint32_t *stale_key,
+                     const int32_t *stale_bbox, void *stream);
 
 /* id / volfrac / weight / velsolid fields (psclattice3d.py:45-50) decoded from the keys. */
 int lbdem_psc_decode(const lbdem_lattice *lat, const uint32_t *key, const float *snap, int32_t *id,

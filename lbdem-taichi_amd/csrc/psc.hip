@@ -28,6 +28,8 @@ struct PscArgs {
   int *bbox;     // 6 per grain: i0, i1, j0, j1, k0, k1 (half-open)
   const float *cell_force;
   double *force_fluid, *moment_fluid;
+  uint32_t *stale_key;     // keys of an earlier mapping, zeroed by the force kernel (may be null)
+  const int *stale_bbox;   // that mapping's boxes
 };
 
 // bounding box of lattice2grains (psclattice3d.py:326-337): [int(xc - r), int(xc + r + 2))
@@ -178,6 +180,17 @@ __global__ __launch_bounds__(kForceThreads) void k_psc_forces(const PscArgs a) {
       T[2] -= d0 * f1 - d1 * f0;
     }
   }
+  if (a.stale_key) {  // the other key buffer, for the next mapping: zero this grain's old box
+    const int *sb = a.stale_bbox + 6 * g;
+    const int si = sb[1] - sb[0], sj = sb[3] - sb[2], sk = sb[5] - sb[4];
+    if (si > 0 && sj > 0 && sk > 0) {
+      const int tot = si * sj * sk;
+      for (int t = threadIdx.x; t < tot; t += kForceThreads) {
+        const int k = sb[4] + t % sk, j = sb[2] + (t / sk) % sj, i = sb[0] + t / (sk * sj);
+        a.stale_key[(i * a.ny + j) * a.nz + k] = 0u;
+      }
+    }
+  }
   constexpr int kWaves = kForceThreads / 64;
   __shared__ double sh[6][kWaves];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -294,7 +307,7 @@ int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_gr
 }
 
 int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u, const uint32_t *key,
-                     const float *cell_force, void *stream) {
+                     const float *cell_force, uint32_t *stale_key, const int32_t *stale_bbox, void *stream) {
   PscArgs a;
   int rc = fill_psc(lat, g, u, a, "lbdem_psc_forces");
   if (rc) return rc;
@@ -302,8 +315,12 @@ int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbde
   LBDEM_REQUIRE(g->n == 0 || (g->position && g->radius && g->force_fluid && g->moment_fluid),
                 "lbdem_psc_forces: grain position, radius, force_fluid and moment_fluid are required");
   if (g->n == 0) return LBDEM_OK;
+  LBDEM_REQUIRE(!stale_key || (stale_bbox && stale_key != key), "lbdem_psc_forces: stale_key needs stale_bbox "
+                "and must differ from key");
   a.key = const_cast<uint32_t *>(key);
   a.cell_force = cell_force;
+  a.stale_key = stale_key;
+  a.stale_bbox = stale_bbox;
   hipLaunchKernelGGL(k_psc_forces, dim3(g->n), dim3(kForceThreads), 0, as_stream(stream), a);
   LBDEM_CHECK_LAUNCH("lbdem_psc_forces");
   return LBDEM_OK;

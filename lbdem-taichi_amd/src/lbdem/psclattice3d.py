@@ -36,7 +36,7 @@ _P = ctypes.POINTER
 _hip.register({
     "lbdem_psc_map": (ctypes.c_int, [_P(_hip.Lattice), _vp, _P(Grains), _P(Units), _vp, _vp, _vp, ctypes.c_int32,
                                      _vp]),
-    "lbdem_psc_forces": (ctypes.c_int, [_P(_hip.Lattice), _P(Grains), _P(Units), _vp, _vp, _vp]),
+    "lbdem_psc_forces": (ctypes.c_int, [_P(_hip.Lattice), _P(Grains), _P(Units), _vp, _vp, _vp, _vp, _vp]),
     "lbdem_psc_decode": (ctypes.c_int, [_P(_hip.Lattice), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 })
 
@@ -69,11 +69,15 @@ class PSCLattice3D(BasicLattice3D):
         self.dem = demslover
         dev = self.device
         ng = max(self.dem.n, 1)
-        self._key = torch.zeros(self.N, dtype=torch.int32, device=dev)  # uint32 bit patterns
+        # Two key arrays (uint32 bit patterns) with their grain boxes, used by turns: the force
+        # reduction after mapping n zeroes the keys of mapping n - 1 in the same launch, so
+        # mapping n + 1 starts from clean keys without a clear pass.
+        self._keys = [torch.zeros(self.N, dtype=torch.int32, device=dev) for _ in range(2)]
+        self._bboxes = [torch.zeros(6 * ng, dtype=torch.int32, device=dev) for _ in range(2)]
+        self._dirty = [False, False]   # keys of a mapping not yet zeroed
+        self._cur = 1                  # buffer of the latest mapping
         self._snap = torch.zeros(12 * ng, dtype=torch.float32, device=dev)
-        self._bbox = torch.zeros(6 * ng, dtype=torch.int32, device=dev)
         self._cell_force = torch.zeros(3 * self.N, dtype=torch.float32, device=dev)
-        self._mapped = False
         self._decoded_version = -1
         shp = (self.Nx, self.Ny, self.Nz)
         self.id = Field(shp, torch.int32, dev, refresh=self._decode)
@@ -85,6 +89,10 @@ class PSCLattice3D(BasicLattice3D):
 
     def _collision(self):
         return _hip.COLL_BGK
+
+    @property
+    def _key(self):
+        return self._keys[self._cur]
 
     def _coupling_args(self, a):
         a.psc_key = self._key.data_ptr()
@@ -110,17 +118,22 @@ class PSCLattice3D(BasicLattice3D):
         """Map grain id, solid fraction, weight and velocity to the lattice (:84-182)."""
         self._prepare()
         g, u = self._structs()
+        b = 1 - self._cur
         _hip.check(_hip.lib().lbdem_psc_map(self._lat_c, _hip.ptr(self._flags), g, u,
-                                            _hip.ptr(self._key), _hip.ptr(self._snap), _hip.ptr(self._bbox),
-                                            1 if self._mapped else 0, _hip.stream()), "lbdem_psc_map")
-        self._mapped = True
+                                            _hip.ptr(self._keys[b]), _hip.ptr(self._snap), _hip.ptr(self._bboxes[b]),
+                                            1 if self._dirty[b] else 0, _hip.stream()), "lbdem_psc_map")
+        self._cur = b
+        self._dirty[b] = True
         self._map_count += 1
 
     def lattice2grains(self):
         """Hydrodynamic force and torque on the grains (:316-358), FP64, physical units."""
         g, u = self._structs()
-        _hip.check(_hip.lib().lbdem_psc_forces(self._lat_c, g, u, _hip.ptr(self._key),
-                                               _hip.ptr(self._cell_force), _hip.stream()), "lbdem_psc_forces")
+        other = 1 - self._cur
+        stale = (_hip.ptr(self._keys[other]), _hip.ptr(self._bboxes[other])) if self._dirty[other] else (None, None)
+        _hip.check(_hip.lib().lbdem_psc_forces(self._lat_c, g, u, _hip.ptr(self._key), _hip.ptr(self._cell_force),
+                                               stale[0], stale[1], _hip.stream()), "lbdem_psc_forces")
+        self._dirty[other] = False
         self.dem.gf.force_fluid.touch()
         self.dem.gf.moment_fluid.touch()
 
