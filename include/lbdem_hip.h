@@ -74,8 +74,9 @@ typedef struct lbdem_lattice {
   double cs_smag;      /* Smagorinsky constant */
   double rho0;         /* shift density of the stored populations */
   double force[3];     /* body force in lattice units: adds rho*w_q*(c_q.F)/cs^2 */
-  int32_t kernel;      /* collide-stream variant: 0 automatic (2 for FP32 storage, 3 for FP64
-                          and 16-bit storage, 1 where the row does not fit), 1 one cell per thread,
+  int32_t kernel;      /* collide-stream variant: 0 automatic (2 for FP32 storage, 3 for FP64,
+                          4 for 16-bit storage, 3 if nz % 4 != 0; 1 where the row does not fit),
+                          1 one cell per thread,
                           2 / 3 / 4 row-tiled (one workgroup per (i, j) row, aligned accesses,
                           z shift of odd steps through LDS) with 1 / 2 / 4 cells per thread;
                           nz must be a multiple of that and at most 1024 times it; 4 is not

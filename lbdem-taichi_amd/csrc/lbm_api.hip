@@ -187,10 +187,15 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
                 "lbdem_lbm_step: VANS needs volfrac, hforce, volfrac_prev, hforce_prev, nu_lu > 0");
   // row-tiled variants: whole rows per workgroup, LDS staging of the z shift (<= 64 KiB);
   // kernel 2 / 3 / 4: one / two / four cells per thread.  Automatic choice (kernel 0), the
-  // fastest measured on MI355X: one cell per thread for FP32, two for FP64 and 16-bit storage.
+  // fastest measured on MI355X (256^3 TGV): one cell per thread for FP32 (4-byte accesses), two
+  // for FP64 (16 bytes), four for 16-bit storage (8 bytes; two where nz is not a multiple of 4).
   int kern = lat->kernel;
   LBDEM_REQUIRE(kern >= 0 && kern <= 4, "lbdem_lbm_step: bad kernel variant %d", kern);
-  if (kern == 0) kern = lat->storage == LBDEM_STORE_F32 ? 2 : 3;
+  if (kern == 0) {
+    if (lat->storage == LBDEM_STORE_F32) kern = 2;
+    else if (lat->storage == LBDEM_STORE_F64) kern = 3;
+    else kern = lat->nz % 4 == 0 ? 4 : 3;
+  }
   const int V = kern == 4 ? 4 : kern == 3 ? 2 : 1;
   const int tsize = lat->storage == LBDEM_STORE_F64 ? 8 : 4;
   const int block = ((lat->nz / V + 63) / 64) * 64;

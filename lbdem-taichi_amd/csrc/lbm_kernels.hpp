@@ -711,27 +711,32 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
   // ---------------------------------------------------------------- gather
   if constexpr (!ODD) {
     if (lane_in) {
+      // all 19 row loads first: a branch between a 16-bit load and its conversion would make
+      // each conversion wait for its load before the next load is issued
       static_for<0, Q>([&](auto qc) {
         constexpr int q = qc;
         T t[V];
         Vec<S, V>::ld(F + (q * N + (size_t)base) + uk0, t);
-        if (q != 0 && ((nm_any >> kOPP[q]) & 1u)) {  // bounced at the wall x - c_q: in the wall's slot
-#pragma unroll
-          for (int v = 0; v < V; ++v) {
-            if ((nmv[v] >> kOPP[q]) & 1u)
-              t[v] = Store<S>::ld(F + kOPP[q] * N + nbr_index(p, i, j, k0 + v, -kCX[q], -kCY[q], -kCZ[q]));
-          }
-        }
 #pragma unroll
         for (int v = 0; v < V; ++v) f[v][q] = t[v];
       });
+      if (nm_any & ~kNbrMoving) {  // bounced at the wall x - c_q last step: in the wall's slot
+        static_for<1, Q>([&](auto qc) {
+          constexpr int q = qc;
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if ((nmv[v] >> kOPP[q]) & 1u)
+              f[v][q] = Store<S>::ld(F + kOPP[q] * N + nbr_index(p, i, j, k0 + v, -kCX[q], -kCY[q], -kCZ[q]));
+          }
+        });
+      }
     }
   } else {
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      const int srow = wrapi(i - cx, p.nx) * p.ny + wrapi(j - cy, p.ny);
-      if (lane_in) {
+    if (lane_in) {  // one basic block for all 19 loads, so they are all issued before any wait
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+        const int srow = wrapi(i - cx, p.nx) * p.ny + wrapi(j - cy, p.ny);
         T t[V];
         Vec<S, V>::ld(F + (kOPP[q] * N + (size_t)srow * nz) + uk0, t);
         if constexpr (cz == 0) {
@@ -741,8 +746,8 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
 #pragma unroll
           for (int v = 0; v < V; ++v) buf[zslot<q>() * ncz + k0 + v] = t[v];
         }
-      }
-    });
+      });
+    }
     __syncthreads();
     if (lane_in) {
       static_for<1, Q>([&](auto qc) {
@@ -813,11 +818,11 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
     }
   } else {
     __syncthreads();  // the gather's LDS reads are done
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = qc;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
-      if (lane_in) {
+    if (lane_in) {
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+        const int trow = wrapi(i + cx, p.nx) * p.ny + wrapi(j + cy, p.ny);
         T t[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) t[v] = f[v][q];
@@ -831,8 +836,8 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
 #pragma unroll
           for (int v = 0; v < V; ++v) buf[zslot<q>() * ncz + k0 + v] = t[v];
         }
-      }
-    });
+      });
+    }
     __syncthreads();
     if (lane_in) {
       static_for<1, Q>([&](auto qc) {
