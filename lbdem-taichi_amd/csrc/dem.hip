@@ -278,7 +278,7 @@ __device__ __forceinline__ void periodic_delta(const lbdem_dem_params &P, const 
 }
 
 // contact law in the local frame (hertz.py:45-153, linear.py:43-152)
-__device__ void contact_law(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j, bool wall,
+__device__ __forceinline__ void contact_law(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j, bool wall,
                             double delta_n, const double *vc, double *shear, double *F) {
   const double *mi = P.mat[0], *mj = P.mat[wall ? 1 : 0];
   const double dt = P.dt;
@@ -352,7 +352,7 @@ __device__ void contact_law(const lbdem_dem_params &P, const lbdem_dem_state &S,
 }
 
 // evaluate an active particle-particle contact held in slot `off` (demsolver.py:683-740)
-__device__ void eval_pair(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j, int off, const double *d,
+__device__ __forceinline__ void eval_pair(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j, int off, const double *d,
                           double dist, double gap) {
   double a[3] = {d[0] / dist, d[1] / dist, d[2] / dist}, R[9], Ri[9];
   rot_to_x(a, R);
@@ -413,7 +413,7 @@ __device__ __forceinline__ bool touching(const lbdem_dem_params &P, const double
 // Phase 1 of resolve (demsolver.py:638-668): pairs with an active history slot are evaluated
 // (gap < 0) or released (gap >= 0).  Returns 1 when the pair needs a new slot.  Only reads
 // slots of other pairs, so it can run for all pairs concurrently.
-__device__ int resolve_existing(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j) {
+__device__ __forceinline__ int resolve_existing(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j) {
   const int K = P.K;
   const int ncf = min(S.cfn[i], K);
   int off = -1;
@@ -435,7 +435,7 @@ __device__ int resolve_existing(const lbdem_dem_params &P, const lbdem_dem_state
 }
 
 // Phase 2: a new contact takes the next slot of i (append_contact_offset, demsolver.py:352-359)
-__device__ void resolve_new(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j) {
+__device__ __forceinline__ void resolve_new(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, int j) {
   const int K = P.K;
   const int slot = atomicAdd(S.cfn + i, 1);
   if (slot >= K) {  // the reference prints and writes cf[-1]; here: counted, no write
@@ -468,7 +468,7 @@ __device__ __forceinline__ void store_wall_mask(const lbdem_dem_state &S, int i,
 
 // resolve_wall + evaluate_wall (demsolver.py:742-838) for particle i; forces added in place.
 // wact: bit w = contact with wall w active (wcf_active), updated here; the caller keeps it.
-__device__ void resolve_walls(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, unsigned &wact) {
+__device__ __forceinline__ void resolve_walls(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, unsigned &wact) {
   // walls within reach, all six tested without branches; the loop below visits only those and
   // the walls with an active contact (usually none)
   unsigned reach = 0u;
@@ -562,7 +562,7 @@ __device__ __forceinline__ void wrap_position(const lbdem_dem_params &P, double 
     }
 }
 
-__device__ void rotate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
+__device__ __forceinline__ void rotate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
   const double dt = P.dt;
   double alpha[3];
   const double *In = S.inertia + 9 * i;
@@ -606,7 +606,7 @@ __device__ void rotate(const lbdem_dem_params &P, const lbdem_dem_state &S, int 
 }
 
 // walls, apply_body_force (demsolver.py:437-448) and update (:514-636) for particle i
-__device__ void walls_and_integrate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, unsigned &wact) {
+__device__ __forceinline__ void walls_and_integrate(const lbdem_dem_params &P, const lbdem_dem_state &S, int i, unsigned &wact) {
   resolve_walls(P, S, i, wact);
   const double dt = P.dt;
 #pragma unroll
@@ -638,7 +638,7 @@ __device__ void walls_and_integrate(const lbdem_dem_params &P, const lbdem_dem_s
 }
 
 // late_clear_state / remove_inactive_contact (demsolver.py:371-386, 431-436)
-__device__ void compact_slots(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
+__device__ __forceinline__ void compact_slots(const lbdem_dem_params &P, const lbdem_dem_state &S, int i) {
   const int K = P.K;
   const int m = min(S.cfn[i], K);
   int act = 0;
