@@ -124,7 +124,20 @@ def cpu_baseline(seed, budget_s=20.0):
                       f"cell x grain, weighted collide, stream, FP64 forces, 10 DEM substeps), FP32, OpenMP"}
 
 
-def tgv_extra(storage, smag="fixed", n=256, steps=20):
+def tgv_variants_study(n=256):
+    """Collide-stream GLUPS of each storage type and row-tiled variant on the 256^3 TGV."""
+    cases = [("f32", "fixed", 2), ("f32", "fixed", 3), ("f32", "lagged", 2), ("bf16", "fixed", 4),
+             ("bf16", "fixed", 3), ("f16", "fixed", 4), ("f16", "fixed", 3), ("f64", "fixed", 3)]
+    out = []
+    for storage, smag, kernel in cases:
+        r = tgv_extra(storage, smag=smag, n=n, kernel=kernel)
+        r.update(storage=storage, smagorinsky=smag, kernel=kernel)
+        print(f"tgv {storage} {smag} kernel {kernel}: {r['mlups']} MLUPS", file=sys.stderr, flush=True)
+        out.append(r)
+    return {"study": "tgv", "lattice": n, "cases": out}
+
+
+def tgv_extra(storage, smag="fixed", n=256, steps=20, kernel=0):
     """Collide-stream MLUPS and HBM fraction on the 256^3 periodic Taylor-Green vortex."""
     import numpy as np
     import torch
@@ -136,6 +149,7 @@ def tgv_extra(storage, smag="fixed", n=256, steps=20):
     v[..., 0] = 0.05 * np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z)
     v[..., 1] = -0.05 * np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z)
     Lt = BasicLattice3D(n, n, n, 1.0 / 0.50382, 1.0, 1.0, 1.0, storage=storage, smagorinsky=smag)
+    Lt.kernel = kernel
     Lt.vel.from_numpy(v)
     Lt.initialize()
     Lt.step(4)
@@ -378,7 +392,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved"], default=None,
+    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv"], default=None,
                     help="run a study instead of the benchmark and print its JSON summary")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 512 MiB three times (reference kernel for HBM byte counters)")
@@ -398,6 +412,9 @@ def main():
         return
     if args.study == "unresolved":
         print(json.dumps(unresolved_study()), flush=True)
+        return
+    if args.study == "tgv":
+        print(json.dumps(tgv_variants_study()), flush=True)
         return
     if args.calibrate_copy:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")

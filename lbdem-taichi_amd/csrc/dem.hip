@@ -875,26 +875,33 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
       }
     }
   }
+  // clear_state of the first substep (later ones are cleared at the end of the one before)
+  for (int i = threadIdx.x; i < n; i += NT) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) forcev[3 * i + a] = momv[3 * i + a] = 0.0;
+  }
+  if (threadIdx.x == 0) nnew = 0;
   __syncthreads();
   for (int sub = 0; sub < nsub; ++sub) {
     // bucket lists are needed by a Verlet-list build and by the per-substep probe
     bool pushed = true;
     if constexpr (RESIDENT) pushed = vl_rebuild || vl_off;  // uniform: stable in this phase
-    // clear_state, own cell hash, push onto the bucket list
-    for (int i = threadIdx.x; i < n; i += NT) {
+    // Barriers: after the bucket push (only when the lists are needed), after the pair phase,
+    // after the new contacts (only if there are any) and after the integration.  clear_state
+    // (forces, moments) and the new-contact count of a substep are reset at the end of the
+    // previous one, or before the loop.
+    if (pushed) {  // uniform: own cell hash, push onto the bucket list
+      for (int i = threadIdx.x; i < n; i += NT) {
+        int32_t c[3];
+        cell_of(P, posv + 3 * i, c);
+        const int32_t h = hash3(c[0], c[1], c[2], P.T);
+        chash[i] = h;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) forcev[3 * i + a] = momv[3 * i + a] = 0.0;
-      if (!pushed) continue;
-      int32_t c[3];
-      cell_of(P, posv + 3 * i, c);
-      const int32_t h = hash3(c[0], c[1], c[2], P.T);
-      chash[i] = h;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) ccell[3 * i + a] = c[a];
-      next[i] = atomicExch(&head[h & (kSmallSlots - 1)], i);
+        for (int a = 0; a < 3; ++a) ccell[3 * i + a] = c[a];
+        next[i] = atomicExch(&head[h & (kSmallSlots - 1)], i);
+      }
+      __syncthreads();
     }
-    if (threadIdx.x == 0) nnew = 0;
-    __syncthreads();
     // candidates of i; contacts already in a history slot are resolved here, new ones listed
     // (the probe count is a template argument of the walk so the hash array stays in registers)
     auto walk = [&](auto np) {
@@ -966,6 +973,13 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
           if (cnt > kNbr) atomicOr(&vl_over, 1);
         }
         __syncthreads();
+        // every thread has read vl_rebuild for this substep; the integration phase may set it
+        // again, after the pair phase's barrier.  vl_off turning 1 here leaves use_lists false
+        // either way (vl_over is 1).
+        if (threadIdx.x == 0) {
+          vl_rebuild = 0;  // lists are current
+          if (vl_over) vl_off = 1;
+        }
       }
       use_lists = !vl_off && !vl_over;  // vl_over is final after the build barrier
     }
@@ -983,18 +997,12 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
       walk(std::integral_constant<int, 8>{});
     }
     __syncthreads();
-    if constexpr (RESIDENT) {
-      if (threadIdx.x == 0 && rebuilt) {
-        vl_rebuild = 0;  // lists are current
-        if (vl_over) vl_off = 1;
-      }
-    }
-    const int nn = min(nnew, newcap);
+    const int nn = min(nnew, newcap);  // uniform: nnew is reset after this phase's reads
     if (threadIdx.x == 0 && nnew > newcap) atomicAdd(S.status + 0, nnew - newcap);
     for (int t = threadIdx.x; t < nn; t += NT) resolve_new(P, S, newlist[2 * t], newlist[2 * t + 1]);
     if (pushed)
       for (int t = threadIdx.x; t < kSmallSlots; t += NT) head[t] = -1;  // lists are rebuilt next substep
-    __syncthreads();
+    if (nn > 0) __syncthreads();  // new contacts add forces that the integration reads
     // integration and slot compaction touch particle i's own data only
     for (int i = threadIdx.x; i < n; i += NT) {
       if constexpr (RESIDENT) {
@@ -1014,7 +1022,13 @@ __global__ __launch_bounds__(NT) void k_dem_small(const lbdem_dem_params P, cons
           if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > hs * hs) atomicOr(&vl_rebuild, 1);
         }
       }
+      if (sub + 1 < nsub) {  // clear_state of the next substep; the last one's forces are kept
+#pragma unroll
+        for (int a = 0; a < 3; ++a) forcev[3 * i + a] = momv[3 * i + a] = 0.0;
+      }
     }
+    // every thread read nnew before this phase (it is 0 unless the new-contact barrier ran)
+    if (threadIdx.x == 0) nnew = 0;
     __syncthreads();
   }
   if constexpr (RESIDENT) {
