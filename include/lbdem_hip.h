@@ -64,6 +64,15 @@ extern "C" {
 #define LBDEM_FLAG_WALL 0x04   /* bounce-back wall (CellType.OBSTACLE) */
 #define LBDEM_FLAG_MOVING 0x08 /* moving bounce-back wall (CellType.VEL_LADD), velocity in vel[] */
 #define LBDEM_FLAG_FLUID 0x10  /* rho,u are refreshed after streaming (CellType.FLUID) */
+#define LBDEM_FLAG_INLET 0x20  /* with MOVING: Ladd velocity inlet (CellType.VEL_INLET_LADD); its
+                                  Ladd term uses this cell's density (step_args.rho) */
+
+/* wet-node boundary cells (lbdem_lbm_boundaries): kind = type | face << 4 */
+#define LBDEM_BC_VEL_ZOUHE 1   /* CellType.VEL_ZOUHE: Zou-He velocity (vel given, rho computed) */
+#define LBDEM_BC_PRE_ZOUHE 2   /* CellType.Pre_ZOUHE: Zou-He pressure (rho given, normal u computed) */
+#define LBDEM_BC_EXIT 3        /* CellType.VEL_EXIT: neighbour's velocity, rho = 1, f = feq */
+#define LBDEM_BC_LADD_INLET 4  /* CellType.VEL_INLET_LADD: rho = inner neighbour's rho */
+/* face: 0 LEFT (x = 0), 1 RIGHT, 2 BOTTOM (y = 0), 3 TOP, 4 BACK (z = 0), 5 FRONT, 7 none */
 
 typedef struct lbdem_lattice {
   int32_t nx, ny, nz;  /* lattice extents */
@@ -104,6 +113,7 @@ typedef struct lbdem_step_args {
   const float *hforce_prev;    /* VANS: force used for the macroscopic velocity */
   int64_t t;                   /* number of steps already taken (AA parity) */
   double nu_lu;                /* base lattice viscosity (1/omega0 - 1/2)/3 (EqIBM, VANS) */
+  const float *rho;            /* cell density field; read at INLET walls (may be NULL without) */
 } lbdem_step_args;
 
 int lbdem_abi_version(void);
@@ -139,6 +149,16 @@ int lbdem_lbm_init(const lbdem_lattice *lat, const lbdem_step_args *a, const flo
  * compute_weight :200-238 evaluated inline), Unresolvedlattice3D.collide + stream
  * (src/unlbdem/vanslattice.py:203-440). */
 int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *stream);
+
+/* Wet-node boundaries, after the step that produced the state after a->t steps: the second
+ * loop of BasicLattice3D.stream (src/lbm3d/lbm_solver3d.py:241-250) over the ncells boundary
+ * cells (cells[e] with kinds[e], LBDEM_BC_* | face << 4): vel_zouHe (:671-820), pre_zouHe
+ * (:823-957), bc_vel_exit (:645-667), bc_ladd_inlet (:516-527).  The Zou-He and exit cells are
+ * ACTIVE (collided and streamed like fluid, not FLUID); their populations are rebuilt in place
+ * and rho (velocity condition, exit) or vel (pressure condition, exit) updated.  Neighbour values
+ * come from the neighbour's populations when it is FLUID, else from rho / vel. */
+int lbdem_lbm_boundaries(const lbdem_lattice *lat, const lbdem_step_args *a, const int32_t *cells,
+                         const uint8_t *kinds, int32_t ncells, float *rho, float *vel, void *stream);
 
 /* rho, vel (AoS) of the post-stream state after t steps, at FLUID cells (all active cells
  * for VANS, with its force/volume-fraction correction).  If `mask` is non-NULL only cells

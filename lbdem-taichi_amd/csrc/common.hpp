@@ -113,6 +113,7 @@ struct KParams {
   void *f;
   const uint8_t *flags;
   const uint32_t *nmask;  // wall-neighbour bits of NEAR cells (bit q: x + c_q is a wall)
+  const float *rho_field;  // cell densities (Ladd term at INLET walls, boundary pass)
   const float *vel;
   void *lag;  // float, or double with FP64 storage
   const uint32_t *psc_key;

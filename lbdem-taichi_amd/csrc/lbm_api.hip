@@ -66,6 +66,7 @@ static int fill_params(const lbdem_lattice *lat, const lbdem_step_args *a, KPara
   p.fz = lat->force[2];
   p.has_force = (lat->force[0] != 0.0 || lat->force[1] != 0.0 || lat->force[2] != 0.0) ? 1 : 0;
   p.t_odd = (int)(a->t & 1);
+  p.rho_field = a->rho;
   return LBDEM_OK;
 }
 
@@ -258,6 +259,26 @@ int lbdem_lbm_write_f(const lbdem_lattice *lat, const lbdem_step_args *a, const 
   p.in = in;
   p.out_f64 = in_f64 ? 1 : 0;
   return launch(t->write_f, p.ncell, p, stream, "lbdem_lbm_write_f");
+}
+
+int lbdem_lbm_boundaries(const lbdem_lattice *lat, const lbdem_step_args *a, const int32_t *cells,
+                         const uint8_t *kinds, int32_t ncells, float *rho, float *vel, void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_boundaries");
+  if (rc) return rc;
+  LBDEM_REQUIRE(ncells >= 0, "lbdem_lbm_boundaries: negative cell count");
+  if (ncells == 0) return LBDEM_OK;
+  LBDEM_REQUIRE(cells && kinds && rho && vel, "lbdem_lbm_boundaries: cells, kinds, rho and vel are required");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_boundaries: bad storage %d", lat->storage);
+  void *args[] = {&p, &cells, &kinds, &ncells, &rho, &vel};
+  hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(t->boundaries), dim3((unsigned)((ncells + 255) / 256)),
+                                 dim3(256), args, 0, as_stream(stream));
+  if (e != hipSuccess) {
+    set_error("lbdem_lbm_boundaries: launch failed: %s", hipGetErrorString(e));
+    return LBDEM_E_HIP;
+  }
+  return LBDEM_OK;
 }
 
 int64_t lbdem_reduce_work_doubles(const lbdem_lattice *lat) {

@@ -414,6 +414,13 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
 
 // Moving-wall (Ladd) momentum term for a population leaving along c_q into wall cell m:
 // the bounced value is f*_q - 6 w_q rho (c_q . u_w)  (src/lbm3d/lbm_solver3d.py:221-224).
+// density of the Ladd term: the bouncing cell's own (VEL_LADD, lbm_solver3d.py:221-224), or the
+// inlet cell's for a Ladd velocity inlet (VEL_INLET_LADD, :225-229)
+template <typename T>
+__device__ __forceinline__ T wall_rho(const KParams &p, uint8_t fm, int m, T rho) {
+  return (fm & LBDEM_FLAG_INLET) ? (T)p.rho_field[m] : rho;
+}
+
 template <int q, typename T>
 __device__ __forceinline__ T ladd_term(const KParams &p, int m, T rho) {
   const T uwx = (T)p.vel[3 * m], uwy = (T)p.vel[3 * m + 1], uwz = (T)p.vel[3 * m + 2];
@@ -461,7 +468,10 @@ __global__ __launch_bounds__(256) void k_step(const KParams p) {
       } else {
         const int m = nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q]);
         T v = f[q];
-        if (((nm >> q) & 1u) && (nm & kNbrMoving) && (p.flags[m] & LBDEM_FLAG_MOVING)) v -= ladd_term<q, T>(p, m, rho);
+        if (((nm >> q) & 1u) && (nm & kNbrMoving)) {
+          const uint8_t fm = p.flags[m];
+          if (fm & LBDEM_FLAG_MOVING) v -= ladd_term<q, T>(p, m, wall_rho(p, fm, m, rho));
+        }
         if constexpr (!ODD) Store<S>::st(F + kOPP[q] * N + c, v);
         else Store<S>::st(F + q * N + m, v);  // into a wall cell too: read there by the next even step
       }
@@ -576,6 +586,159 @@ __global__ __launch_bounds__(256) void k_write_f(const KParams p) {
     f[q] = (T)(v - kW[q] * p.rho0);
   });
   scatter_state<S>(p, F, c, fl, p.t_odd != 0, f);
+}
+
+// ---------------------------------------------------------------------------------------
+// Wet-node boundaries after a step (the second loop of BasicLattice3D.stream,
+// src/lbm3d/lbm_solver3d.py:241-250), one thread per boundary cell.  The face with normal axis a
+// and inward sign s: the populations with c_a = -s arrived from inside; those with c_a = s are
+// rebuilt from their opposites (vel_zouHe :671-820, pre_zouHe :823-957), in the reference's order
+// of terms.  A boundary cell's populations are read and written at the state's parity through
+// gather / scatter_state, which touch only that cell's own populations.
+// ---------------------------------------------------------------------------------------
+__constant__ int kZhZero[3][9] = {{0, 3, 4, 5, 6, 15, 16, 17, 18},
+                                  {0, 1, 2, 5, 6, 11, 12, 13, 14},
+                                  {0, 1, 2, 3, 4, 7, 8, 9, 10}};
+__constant__ int kZhIn[6][5] = {{2, 8, 10, 12, 14}, {1, 7, 9, 11, 13}, {4, 8, 9, 16, 18},
+                                {3, 7, 10, 15, 17}, {6, 12, 13, 16, 17}, {5, 11, 14, 15, 18}};
+// [axis][transverse axis 0/1][c_t = +1 / -1][3], transverse axes in x, y, z order
+__constant__ int kZhN[3][2][2][3] = {{{{3, 15, 17}, {4, 18, 16}}, {{5, 18, 15}, {6, 17, 16}}},
+                                     {{{1, 11, 13}, {2, 14, 12}}, {{5, 11, 14}, {6, 13, 12}}},
+                                     {{{1, 7, 9}, {2, 10, 8}}, {{3, 7, 10}, {4, 9, 8}}}};
+__constant__ int kBcCx[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+__constant__ int kBcCy[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+__constant__ int kBcCz[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+__constant__ int kBcOpp[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+// the known populations' sum of the density formula: c_a = 0 ones plus twice the incoming ones
+template <typename T>
+__device__ T zh_sum_known(const T *f, int face) {
+  const int a = face >> 1;
+  T s0 = T(0), s1 = T(0);
+  for (int m = 0; m < 9; ++m) s0 += f[kZhZero[a][m]];
+  for (int m = 0; m < 5; ++m) s1 += f[kZhIn[face][m]];
+  return s0 + T(2) * s1;
+}
+
+// the five unknown populations for density rho and velocity u; the pressure condition uses only
+// the normal velocity and no rho u_t term in N_t
+template <typename T>
+__device__ void zh_rebuild(T *f, int face, T rho, const T *u, bool pressure) {
+  const int a = face >> 1, s = (face & 1) ? -1 : 1;
+  const int ts[2] = {a == 0 ? 1 : 0, a == 2 ? 1 : 2};
+  T N[2];
+  for (int t = 0; t < 2; ++t) {
+    const int *pp = kZhN[a][t][0], *mm = kZhN[a][t][1];
+    N[t] = T(0.5) * (f[pp[0]] + f[pp[1]] + f[pp[2]] - (f[mm[0]] + f[mm[1]] + f[mm[2]]));
+    if (!pressure) N[t] = N[t] - T(1.0 / 3.0) * rho * u[ts[t]];
+  }
+  for (int q = 1; q < Q; ++q) {
+    const int c[3] = {kBcCx[q], kBcCy[q], kBcCz[q]};
+    if (c[a] != s) continue;
+    T cu = T(0);
+    int diag = -1;
+    for (int b = 0; b < 3; ++b) {
+      if (c[b] == 0) continue;
+      if (b != a) diag = b;
+      if (pressure && b != a) continue;
+      cu += T(c[b]) * u[b];
+    }
+    if (diag < 0) f[q] = f[kBcOpp[q]] + T(1.0 / 3.0) * rho * cu;
+    else f[q] = f[kBcOpp[q]] + T(1.0 / 6.0) * rho * cu - T(c[diag]) * N[diag == ts[0] ? 0 : 1];
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_boundaries(const KParams p, const int32_t *cells, const uint8_t *kinds,
+                                                    int ncells, float *rho_f, float *vel_f) {
+  using T = typename Store<S>::T;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= ncells) return;
+  const int c = cells[e];
+  const int type = kinds[e] & 15, face = kinds[e] >> 4;
+  S *F = reinterpret_cast<S *>(p.f);
+  const bool odd = p.t_odd != 0;
+  int i, j, k;
+  cell_ijk(p, c, i, j, k);
+  const bool has_face = face <= 5;
+  const int a = face >> 1, s = (face & 1) ? -1 : 1;
+  const int m = has_face ? nbr_index(p, i, j, k, a == 0 ? s : 0, a == 1 ? s : 0, a == 2 ? s : 0) : c;
+  // rho and u of the inner neighbour m: from its populations where it is a fluid cell (the
+  // reference refreshed them in the first loop), else from the fields
+  auto inner = [&](T &r, T (&u)[3]) {
+    const uint8_t fm = p.flags[m];
+    if ((fm & LBDEM_FLAG_FLUID) && (fm & LBDEM_FLAG_ACTIVE)) {
+      T g[Q];
+      gather<S>(p, F, m, fm, odd, g);
+      T dr = T(0), jx = T(0), jy = T(0), jz = T(0);
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = qc;
+        dr += g[q];
+        acc<kCX[q]>(jx, g[q]);
+        acc<kCY[q]>(jy, g[q]);
+        acc<kCZ[q]>(jz, g[q]);
+      });
+      r = T(p.rho0) + dr;
+      u[0] = jx / r;
+      u[1] = jy / r;
+      u[2] = jz / r;
+    } else {
+      r = (T)rho_f[m];
+      for (int b = 0; b < 3; ++b) u[b] = (T)vel_f[3 * m + b];
+    }
+  };
+  if (type == LBDEM_BC_LADD_INLET) {  // bc_ladd_inlet: density from the inner neighbour
+    if (!has_face) return;
+    T r, u[3];
+    inner(r, u);
+    rho_f[c] = (float)r;
+    return;
+  }
+  const uint8_t fl = p.flags[c];
+  if (!(fl & LBDEM_FLAG_ACTIVE)) return;
+  T f[Q];
+  if (type == LBDEM_BC_EXIT) {  // bc_vel_exit: the neighbour's velocity, rho = 1, f = feq
+    T u[3];
+    if (has_face) {
+      T r;
+      inner(r, u);
+      for (int b = 0; b < 3; ++b) vel_f[3 * c + b] = (float)u[b];
+    }
+    for (int b = 0; b < 3; ++b) u[b] = (T)vel_f[3 * c + b];
+    rho_f[c] = 1.0f;
+    const T usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = qc;
+      const T cu = cdot<q>(u[0], u[1], u[2]);
+      f[q] = T(kW[q]) * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq) - T(kW[q] * p.rho0);
+    });
+    scatter_state<S>(p, F, c, fl, odd, f);
+    return;
+  }
+  if (!has_face) return;  // Zou-He without a face: the reference's if-chain does nothing
+  gather<S>(p, F, c, fl, odd, f);
+  T g[Q];  // unshifted
+  for (int q = 0; q < Q; ++q) g[q] = f[q] + T(kW[q] * p.rho0);
+  T u[3];
+  if (type == LBDEM_BC_VEL_ZOUHE) {
+    for (int b = 0; b < 3; ++b) u[b] = (T)vel_f[3 * c + b];
+    const T coeff = s > 0 ? T(1) / (T(1) - u[a]) : T(1) / (T(1) + u[a]);
+    const T r = coeff * zh_sum_known(g, face);
+    rho_f[c] = (float)r;
+    zh_rebuild(g, face, r, u, false);
+  } else if (type == LBDEM_BC_PRE_ZOUHE) {
+    const T r = (T)rho_f[c];
+    const T sk = zh_sum_known(g, face);
+    const T un = s > 0 ? T(1) - sk / r : sk / r - T(1);
+    vel_f[3 * c + a] = (float)un;
+    u[0] = u[1] = u[2] = T(0);
+    u[a] = un;
+    zh_rebuild(g, face, r, u, true);
+  } else {
+    return;
+  }
+  for (int q = 0; q < Q; ++q) f[q] = g[q] - T(kW[q] * p.rho0);
+  scatter_state<S>(p, F, c, fl, odd, f);
 }
 
 // per-workgroup FP64 partial sums: rho, rho u (3), rho |u|^2 / 2
@@ -797,7 +960,8 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
     constexpr int q = decltype(qc)::value;
     if (((nmv[v] >> q) & 1u) && (nmv[v] & kNbrMoving)) {
       const int m = nbr_index(p, i, j, k0 + v, kCX[q], kCY[q], kCZ[q]);
-      if (p.flags[m] & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, rho[v]);
+      const uint8_t fm = p.flags[m];
+      if (fm & LBDEM_FLAG_MOVING) val -= ladd_term<q, T>(p, m, wall_rho(p, fm, m, rho[v]));
     }
   };
 

@@ -5,6 +5,7 @@
 namespace lbdem {
 
 using KernelFn = void (*)(const KParams);
+using BcKernelFn = void (*)(const KParams, const int32_t *, const uint8_t *, int, float *, float *);
 
 struct LbmKernels {
   KernelFn step[3][4][2];  // [collision][coupling][odd]; nullptr = unsupported combination
@@ -15,6 +16,7 @@ struct LbmKernels {
   KernelFn macros[4];      // [coupling]
   KernelFn read_f;
   KernelFn write_f;
+  BcKernelFn boundaries;
   KernelFn reduce_partial;
 };
 
@@ -105,6 +107,7 @@ const LbmKernels &lbm_kernels_f64();
       k.macros[LBDEM_COUPLE_VANS] = k_macros<S, LBDEM_COUPLE_VANS>;                             \
       k.read_f = k_read_f<S>;                                                                   \
       k.write_f = k_write_f<S>;                                                                 \
+      k.boundaries = k_boundaries<S>;                                                           \
       k.reduce_partial = k_reduce_partial<S>;                                                   \
       return k;                                                                                 \
     }();                                                                                        \

@@ -15,7 +15,8 @@ STORE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
 STORE_DTYPE = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 COLL_BGK, COLL_SMAG_LAGGED, COLL_SMAG_FIXED = 0, 1, 2
 COUPLE_NONE, COUPLE_PSC, COUPLE_EQIBM, COUPLE_VANS = 0, 1, 2, 3
-FLAG_ACTIVE, FLAG_NEAR, FLAG_WALL, FLAG_MOVING, FLAG_FLUID = 0x01, 0x02, 0x04, 0x08, 0x10
+FLAG_ACTIVE, FLAG_NEAR, FLAG_WALL, FLAG_MOVING, FLAG_FLUID, FLAG_INLET = 0x01, 0x02, 0x04, 0x08, 0x10, 0x20
+BC_VEL_ZOUHE, BC_PRE_ZOUHE, BC_EXIT, BC_LADD_INLET = 1, 2, 3, 4
 
 _vp = ctypes.c_void_p
 
@@ -35,7 +36,7 @@ class StepArgs(ctypes.Structure):
                 ("psc_key", _vp), ("psc_grain", _vp), ("psc_force", _vp),
                 ("volfrac", _vp), ("velsolid", _vp), ("hforce", _vp),
                 ("volfrac_prev", _vp), ("hforce_prev", _vp),
-                ("t", ctypes.c_int64), ("nu_lu", ctypes.c_double)]
+                ("t", ctypes.c_int64), ("nu_lu", ctypes.c_double), ("rho", _vp)]
 
 
 _lib = None
@@ -62,6 +63,7 @@ def lib():
         "lbdem_reduce_work_doubles": (ctypes.c_int64, [P(Lattice)]),
         "lbdem_lbm_reduce": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, _vp]),
         "lbdem_lbm_prepare_flags": (ctypes.c_int, [P(Lattice), _vp, _vp, _vp]),
+        "lbdem_lbm_boundaries": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, ctypes.c_int32, _vp, _vp, _vp]),
     }
     sig.update(_EXTRA_SIGNATURES)
     for name, (res, args) in sig.items():

@@ -153,6 +153,7 @@ class BasicLattice3D:
         a.lag = self._lag.data_ptr() if self._lag is not None else None
         a.t = self.t
         a.nu_lu = (1.0 / self.omega0 - 0.5) / 3.0
+        a.rho = self.rho.dev.data_ptr()
         self._coupling_args(a)
         return ctypes.byref(a)
 
@@ -162,19 +163,39 @@ class BasicLattice3D:
     def _build_flags(self):
         ct = self.CT.to_numpy().astype(np.int64)
         kind = ct & _TYPE_BITS
+        wet = {CellType.VEL_ZOUHE: _hip.BC_VEL_ZOUHE, CellType.Pre_ZOUHE: _hip.BC_PRE_ZOUHE,
+               CellType.VEL_EXIT: _hip.BC_EXIT, CellType.VEL_INLET_LADD: _hip.BC_LADD_INLET}
         supported = (kind == CellType.FLUID) | (kind == CellType.OBSTACLE) | (kind == CellType.VEL_LADD)
+        for t in wet:
+            supported |= kind == t
         if not supported.all():
             bad = np.unique(kind[~supported])
             raise NotImplementedError(
                 f"cell types {bad.tolist()} are not supported by the fused kernel (supported: FLUID, "
-                "OBSTACLE, VEL_LADD; wet-node and free-slip boundaries are not implemented)")
+                "OBSTACLE, VEL_LADD, VEL_ZOUHE, Pre_ZOUHE, VEL_EXIT, VEL_INLET_LADD; free-slip is not "
+                "implemented)")
         active = (ct & self._skip_types) == 0
         wall = kind == CellType.OBSTACLE
-        moving = kind == CellType.VEL_LADD
+        inlet = kind == CellType.VEL_INLET_LADD
+        if (inlet & active).any():
+            raise NotImplementedError("VEL_INLET_LADD cells must be among the skipped cell types")
+        moving = (kind == CellType.VEL_LADD) | inlet
         fluid = kind == CellType.FLUID
         flags = (active * _hip.FLAG_ACTIVE | wall * _hip.FLAG_WALL | moving * _hip.FLAG_MOVING |
-                 fluid * _hip.FLAG_FLUID).astype(np.uint8)
+                 inlet * _hip.FLAG_INLET | fluid * _hip.FLAG_FLUID).astype(np.uint8)
         self._flags.copy_(torch.from_numpy(flags.reshape(-1)))
+        # wet-node boundary cells, applied after every step: type | face << 4, the face being the
+        # first of LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT set (7: none)
+        bc_type = np.zeros(kind.shape, dtype=np.int64)
+        for t, code in wet.items():
+            bc_type[kind == t] = code
+        face = np.full(kind.shape, 7, dtype=np.int64)
+        for f, bit in reversed(list(enumerate((CellType.LEFT, CellType.RIGHT, CellType.BOTTOM, CellType.TOP,
+                                                CellType.BACK, CellType.FRONT)))):
+            face[(ct & bit) != 0] = f
+        cells = np.nonzero(bc_type.reshape(-1))[0]
+        self._bc_cells = torch.from_numpy(cells.astype(np.int32)).to(self.device)
+        self._bc_kinds = torch.from_numpy((bc_type | face << 4).reshape(-1)[cells].astype(np.uint8)).to(self.device)
         dims = _hip.Lattice()
         dims.nx, dims.ny, dims.nz = self.Nx, self.Ny, self.Nz
         # NEAR bits and wall-neighbour masks are derived on the device
@@ -250,6 +271,14 @@ class BasicLattice3D:
             e1.record()
             ev.append((e0, e1))
         self.t += 1
+        nbc = int(self._bc_cells.numel())
+        if nbc:  # wet-node boundaries on the post-stream state (lbm_solver3d.py:241-250)
+            _hip.check(_hip.lib().lbdem_lbm_boundaries(self._lat_c, self._args_c(), _hip.ptr(self._bc_cells),
+                                                       _hip.ptr(self._bc_kinds), nbc, _hip.ptr(self.rho.dev),
+                                                       _hip.ptr(self.vel.dev), _hip.stream()),
+                       "lbdem_lbm_boundaries")
+            self.rho.touch()
+            self.vel.touch()
         if self._lag is not None:
             self.omega.touch()
 

@@ -190,6 +190,9 @@ void SFX(lbo_stream)(int nx, int ny, int nz, const int32_t *CT, int32_t skip, co
           } else if (t & CT_VEL_LADD) {
             const REAL cu = CX[q] * vel[3 * m] + CY[q] * vel[3 * m + 1] + CZ[q] * vel[3 * m + 2];
             f[Q * c + q] = fpc[Q * c + QINV[q]] + 2.0 * W[q] * rho[c] * cu / (1.0 / 3.0);
+          } else if (t & CT_VEL_INLET_LADD) {  /* the inlet cell's density (:225-229) */
+            const REAL cu = CX[q] * vel[3 * m] + CY[q] * vel[3 * m + 1] + CZ[q] * vel[3 * m + 2];
+            f[Q * c + q] = fpc[Q * c + QINV[q]] + 2.0 * W[q] * rho[m] * cu / (1.0 / 3.0);
           }
           /* other neighbour types leave f[q] unchanged (lbm_solver3d.py:216-236) */
         }
@@ -218,6 +221,124 @@ void SFX(lbo_stream)(int nx, int ny, int nz, const int32_t *CT, int32_t skip, co
 /* PSC grain mapping, PSCLattice3D.grains2lattice (src/lbdem/psclattice3d.py:84-182).
  * grains: per grain [x, y, z, vx, vy, vz, wx, wy, wz, R] (physical units).  Outputs id (-1 =
  * none), volfrac, weight, velsolid (lattice units).  omega: per-cell omega field.          */
+/* Wet-node boundaries: the second loop of BasicLattice3D.stream (src/lbm3d/lbm_solver3d.py:241-250),
+ * after every cell has streamed: vel_zouHe (:671-820), bc_vel_exit (:645-667), pre_zouHe
+ * (:823-957), bc_ladd_inlet (:516-527).  The face of a boundary cell is its first set bit among
+ * LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT (the reference's if-chain).  Zou-He for the face with
+ * normal axis a: the populations with c_a = -s (s = +1 on the low face) are the known incoming
+ * ones, those with c_a = s are rebuilt from their opposites. */
+#define CT_RIGHT 4096
+#define CT_LEFT 8192
+#define CT_TOP 16384
+#define CT_BOTTOM 32768
+#define CT_FRONT 65536
+#define CT_BACK 131072
+
+/* face index 0..5 = LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT; -1 none */
+static int bc_face(int32_t ct) {
+  if (ct & CT_LEFT) return 0;
+  if (ct & CT_RIGHT) return 1;
+  if (ct & CT_BOTTOM) return 2;
+  if (ct & CT_TOP) return 3;
+  if (ct & CT_BACK) return 4;
+  if (ct & CT_FRONT) return 5;
+  return -1;
+}
+
+/* The terms of each face in the reference's order: the c_a = 0 populations, the c_a = -s
+ * ones (doubled in the density sum), then per transverse axis t the c_t = +1 and c_t = -1
+ * populations with c_a = 0 of its N_t term. */
+static const int ZH_ZERO[3][9] = {{0, 3, 4, 5, 6, 15, 16, 17, 18},
+                                  {0, 1, 2, 5, 6, 11, 12, 13, 14},
+                                  {0, 1, 2, 3, 4, 7, 8, 9, 10}};
+static const int ZH_IN[6][5] = {{2, 8, 10, 12, 14}, {1, 7, 9, 11, 13}, {4, 8, 9, 16, 18},
+                                {3, 7, 10, 15, 17}, {6, 12, 13, 16, 17}, {5, 11, 14, 15, 18}};
+/* [axis][transverse 0/1][plus/minus][3], transverse axes in (x, y, z) order skipping a */
+static const int ZH_N[3][2][2][3] = {
+    {{{3, 15, 17}, {4, 18, 16}}, {{5, 18, 15}, {6, 17, 16}}},
+    {{{1, 11, 13}, {2, 14, 12}}, {{5, 11, 14}, {6, 13, 12}}},
+    {{{1, 7, 9}, {2, 10, 8}}, {{3, 7, 10}, {4, 9, 8}}}};
+
+static REAL zh_sum_known(const REAL *fc, int face) {
+  const int a = face / 2;
+  REAL s0 = 0, s1 = 0;
+  for (int m = 0; m < 9; ++m) s0 += fc[ZH_ZERO[a][m]];
+  for (int m = 0; m < 5; ++m) s1 += fc[ZH_IN[face][m]];
+  return s0 + 2.0 * s1;
+}
+
+/* rebuilds the 5 unknown populations of face `face` for density rho and velocity u (u_t = 0 and
+ * no rho u_t term in N_t for the pressure condition, `pressure` != 0) */
+static void zh_rebuild(REAL *fc, int face, REAL rho, const REAL *u, int pressure) {
+  const int a = face / 2, s = (face % 2 == 0) ? 1 : -1;
+  const int ts[2] = {a == 0 ? 1 : 0, a == 2 ? 1 : 2};
+  REAL N[2];
+  for (int t = 0; t < 2; ++t) {
+    const int *pp = ZH_N[a][t][0], *mm = ZH_N[a][t][1];
+    N[t] = 0.5 * (fc[pp[0]] + fc[pp[1]] + fc[pp[2]] - (fc[mm[0]] + fc[mm[1]] + fc[mm[2]]));
+    if (!pressure) N[t] = N[t] - 1.0 / 3.0 * rho * u[ts[t]];
+  }
+  for (int q = 1; q < Q; ++q) {
+    const int c[3] = {CX[q], CY[q], CZ[q]};
+    if (c[a] != s) continue;
+    REAL cu = 0.0;
+    int diag = -1;
+    for (int b = 0; b < 3; ++b) {
+      if (c[b] == 0) continue;
+      if (pressure && b != a) { diag = b; continue; }
+      cu += c[b] * u[b];
+      if (b != a) diag = b;
+    }
+    if (diag < 0) {
+      fc[q] = fc[QINV[q]] + (1.0 / 3.0) * rho * cu;
+    } else {
+      const int t = (diag == ts[0]) ? 0 : 1;
+      fc[q] = fc[QINV[q]] + 1.0 / 6.0 * rho * cu - c[diag] * N[t];
+    }
+  }
+}
+
+void SFX(lbo_boundaries)(int nx, int ny, int nz, const int32_t *CT, REAL *f, REAL *rho, REAL *vel) {
+  /* neighbour values are read from cells the loop does not change (FLUID interior cells) */
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const long c = IDX(i, j, k);
+        const int32_t ct = CT[c];
+        const int face = bc_face(ct);
+        REAL *fc = f + Q * c;
+        if (ct & CT_VEL_ZOUHE) {
+          if (face < 0) continue;
+          const int a = face / 2;
+          const REAL ua = vel[3 * c + a];
+          const REAL coeff = (face % 2 == 0) ? 1 / (1 - ua) : 1 / (1 + ua);
+          rho[c] = coeff * zh_sum_known(fc, face);
+          zh_rebuild(fc, face, rho[c], vel + 3 * c, 0);
+        } else if (ct & CT_VEL_EXIT) {
+          if (face >= 0) {
+            static const int DI[6] = {1, -1, 0, 0, 0, 0}, DJ[6] = {0, 0, 1, -1, 0, 0}, DK[6] = {0, 0, 0, 0, 1, -1};
+            const long m = IDX(i + DI[face], j + DJ[face], k + DK[face]);
+            for (int b = 0; b < 3; ++b) vel[3 * c + b] = vel[3 * m + b];
+          }
+          rho[c] = 1.0;
+          feq_of(rho[c], vel + 3 * c, fc);
+        } else if (ct & CT_PRE_ZOUHE) {
+          if (face < 0) continue;
+          const int a = face / 2;
+          const REAL r = rho[c];
+          if (face % 2 == 0) vel[3 * c + a] = 1.0 - zh_sum_known(fc, face) / r;
+          else vel[3 * c + a] = zh_sum_known(fc, face) / r - 1.0;
+          REAL u[3] = {0, 0, 0};
+          u[a] = vel[3 * c + a];
+          zh_rebuild(fc, face, r, u, 1);
+        } else if (ct & CT_VEL_INLET_LADD) {
+          if (face < 0) continue;
+          static const int DI[6] = {1, -1, 0, 0, 0, 0}, DJ[6] = {0, 0, 1, -1, 0, 0}, DK[6] = {0, 0, 0, 0, 1, -1};
+          rho[c] = rho[IDX(i + DI[face], j + DJ[face], k + DK[face])];
+        }
+      }
+}
+
 void SFX(lbo_psc_map)(int nx, int ny, int nz, const int32_t *CT, int32_t ngrain, const double *grains,
                       const double *dmin, double dx, double dt, const REAL *omega, int32_t *id, REAL *volfrac,
                       REAL *weight, REAL *velsolid) {

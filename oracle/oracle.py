@@ -14,6 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbdem_oracle.so")
 
 FLUID, OBSTACLE, FREE_SLIP, VEL_LADD, VEL_INLET_LADD = 1, 8, 16, 32, 262144
+VEL_ZOUHE, VEL_EXIT, PRE_ZOUHE = 64, 128, 256                      # lbmutils.py:69-97
+RIGHT, LEFT, TOP, BOTTOM, FRONT, BACK = 4096, 8192, 16384, 32768, 65536, 131072
+WET_NODE = VEL_ZOUHE | VEL_EXIT | PRE_ZOUHE | VEL_INLET_LADD
 SKIP_BASIC = OBSTACLE | VEL_LADD | FREE_SLIP | VEL_INLET_LADD   # lbm_solver3d.py:183
 SKIP_FORCED = OBSTACLE | VEL_LADD | FREE_SLIP                   # lbm_foecelover.py:45, psclattice3d.py:204
 
@@ -128,6 +131,8 @@ class OracleLattice:
         _fn("lbo_stream", self.real)(*self._g(), P(self.CT), ctypes.c_int32(self.skip), P(self.fpc), P(self.f),
                                      P(self.rho), P(self.vel), ctypes.c_int32(vans),
                                      P(self.volfrac) if vans else None, P(self.hydroforce) if vans else None)
+        if (self.CT & WET_NODE).any():  # the boundary loop after streaming (lbm_solver3d.py:241-250)
+            _fn("lbo_boundaries", self.real)(*self._g(), P(self.CT), P(self.f), P(self.rho), P(self.vel))
 
     def step(self, n=1):
         for _ in range(n):

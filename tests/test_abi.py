@@ -45,7 +45,7 @@ def test_python_binding_declares_every_header_function():
     known = set(H._EXTRA_SIGNATURES) | {
         "lbdem_abi_version", "lbdem_last_error", "lbdem_lbm_init", "lbdem_lbm_step", "lbdem_lbm_macros",
         "lbdem_lbm_read_f", "lbdem_lbm_write_f", "lbdem_reduce_work_doubles", "lbdem_lbm_reduce",
-        "lbdem_lbm_prepare_flags"}
+        "lbdem_lbm_prepare_flags", "lbdem_lbm_boundaries"}
     missing = [n for n in declared_functions() if n not in known]
     assert not missing, f"no ctypes prototype for {missing}"
 
@@ -68,9 +68,12 @@ def test_header_constants_match_python_binding():
     text = open(HEADER).read()
     consts = {m.group(1): int(m.group(2), 0) for m in re.finditer(r"#define\s+(LBDEM_[A-Z0-9_]+)\s+(0x[0-9a-fA-F]+|\d+)",
                                                                       text)}
-    for name in ("ACTIVE", "NEAR", "WALL", "MOVING", "FLUID"):
+    flags = ("ACTIVE", "NEAR", "WALL", "MOVING", "FLUID", "INLET")
+    for name in flags:
         assert consts[f"LBDEM_FLAG_{name}"] == getattr(H, f"FLAG_{name}"), name
-    assert len({consts[f"LBDEM_FLAG_{n}"] for n in ("ACTIVE", "NEAR", "WALL", "MOVING", "FLUID")}) == 5
+    assert len({consts[f"LBDEM_FLAG_{n}"] for n in flags}) == len(flags)
+    for name in ("VEL_ZOUHE", "PRE_ZOUHE", "EXIT", "LADD_INLET"):
+        assert consts[f"LBDEM_BC_{name}"] == getattr(H, f"BC_{name}"), name
     for key, code in H.STORE.items():
         assert consts[f"LBDEM_STORE_{key.upper()}"] == code, key
     for name in ("COLL_BGK", "COLL_SMAG_LAGGED", "COLL_SMAG_FIXED", "COUPLE_NONE", "COUPLE_PSC", "COUPLE_EQIBM",
