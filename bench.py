@@ -124,6 +124,45 @@ def cpu_baseline(seed, budget_s=20.0):
                       f"cell x grain, weighted collide, stream, FP64 forces, 10 DEM substeps), FP32, OpenMP"}
 
 
+def channel_study(nx=256, ny=128, nz=128, steps=100):
+    """Wet-node boundaries at scale: a 256 x 128 x 128 FP32 channel, walls at y = 0 and ny - 1,
+    periodic z, Zou-He velocity inlet (parabolic, 0.03 lu peak) at x = 0 and Zou-He pressure
+    outlet (rho = 1) at x = nx - 1; collide-stream and boundary-pass times from HIP events. The
+    peak velocity is sampled at x = nx / 16, which the start-up pressure wave has passed."""
+    import numpy as np
+    import torch
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    from src.lbm3d.lbmutils import CellType
+    L = BasicLattice3D(nx, ny, nz, 1.0 / 0.55, 1.0, 1.0, 1.0, storage="f32", smagorinsky="fixed")
+    ct = np.full((nx, ny, nz), CellType.FLUID, dtype=np.int32)
+    ct[:, 0, :] = ct[:, -1, :] = CellType.OBSTACLE
+    ct[0, 1:-1, :] = CellType.VEL_ZOUHE | CellType.LEFT
+    ct[-1, 1:-1, :] = CellType.Pre_ZOUHE | CellType.RIGHT
+    y = np.arange(ny)
+    prof = np.clip(4 * 0.03 * (y - 0.5) * (ny - 1.5 - y) / (ny - 2.0) ** 2, 0, None)
+    v = np.zeros((nx, ny, nz, 3), np.float32)
+    v[0, :, :, 0] = prof[:, None]
+    L.CT.from_numpy(ct)
+    L.vel.from_numpy(v)
+    L.initialize()
+    L.step(10)
+    torch.cuda.synchronize()
+    L.step_events = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    L.step(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    k_ms = sum(a.elapsed_time(b) for a, b in L.step_events) / len(L.step_events)
+    L.step_events = None
+    ux = L.vel.to_numpy()[nx // 16, 1:-1, nz // 2, 0]
+    return {"study": "channel", "lattice": [nx, ny, nz], "boundary_cells": int(L._bc_cells.numel()),
+            "ms_per_step": round(ms, 4), "mlups": round(nx * ny * nz / ms / 1e3, 1),
+            "collide_stream_ms": round(k_ms, 4), "boundary_pass_and_launch_ms": round(ms - k_ms, 4),
+            "peak_u_at_x16": round(float(ux.max()), 5)}
+
+
 def tgv_variants_study(n=256):
     """Collide-stream GLUPS of each storage type and row-tiled variant on the 256^3 TGV."""
     cases = [("f32", "fixed", 2), ("f32", "fixed", 3), ("f32", "lagged", 2), ("bf16", "fixed", 4),
@@ -392,7 +431,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv"], default=None,
+    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv", "channel"], default=None,
                     help="run a study instead of the benchmark and print its JSON summary")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 512 MiB three times (reference kernel for HBM byte counters)")
@@ -415,6 +454,9 @@ def main():
         return
     if args.study == "tgv":
         print(json.dumps(tgv_variants_study()), flush=True)
+        return
+    if args.study == "channel":
+        print(json.dumps(channel_study()), flush=True)
         return
     if args.calibrate_copy:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
