@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 3
+#define LBDEM_ABI_VERSION 4
 
 /* return codes */
 #define LBDEM_OK 0
@@ -55,6 +55,8 @@ extern "C" {
 #define LBDEM_COUPLE_PSC 1   /* partially saturated cells, resolved grains */
 #define LBDEM_COUPLE_EQIBM 2 /* particle-equivalence IBM, Tenneti weight */
 #define LBDEM_COUPLE_VANS 3  /* volume-averaged NS with Guo-type force */
+#define LBDEM_COUPLE_HYBRID 4 /* coarse grains as PSC (keys) and fine grains as EqIBM (fractions)
+                                 in one collision, f + beta Os1 + B Os2 + (1-B-beta) Of */
 
 /* per-cell flag byte */
 #define LBDEM_FLAG_ACTIVE 0x01 /* collided and streamed (not OBSTACLE / VEL_LADD); every cell
@@ -147,7 +149,9 @@ int lbdem_lbm_init(const lbdem_lattice *lat, const lbdem_step_args *a, const flo
  * and reduction are lbdem_psc_map / lbdem_psc_forces),
  * EqIMBlattice3D.collide (src/unlbdem/eqlattice.py:243-303, with the Tenneti weight of
  * compute_weight :200-238 evaluated inline), Unresolvedlattice3D.collide + stream
- * (src/unlbdem/vanslattice.py:203-440). */
+ * (src/unlbdem/vanslattice.py:203-440), HybridLattice3D.collide (src/hybridlbdem/
+ * hybridlattice.py:433-593: coarse keys in psc_key / psc_grain, fine fractions in volfrac /
+ * velsolid, the momentum exchange of the coarse part into psc_force). */
 int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *stream);
 
 /* Wet-node boundaries, after the step that produced the state after a->t steps: the second
@@ -230,6 +234,43 @@ int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbde
 /* id / volfrac / weight / velsolid fields (psclattice3d.py:45-50) decoded from the keys. */
 int lbdem_psc_decode(const lbdem_lattice *lat, const uint32_t *key, const float *snap, int32_t *id,
                      float *volfrac, float *weight, float *velsolid, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* hybrid resolved / unresolved coupling (src/hybridlbdem)                                */
+/* ------------------------------------------------------------------------------------ */
+/* Coarse-grain mapping, HybridLattice3D.map_coarse_grains (src/hybridlbdem/hybridlattice.py:
+ * 173-234): lbdem_psc_map over the grains with 2R > dx only (the others get empty boxes).  The
+ * PSC weight B is not mapped: the hybrid collision evaluates it with the step's own omega
+ * (_psc_compute_weight :236-244).  Coarse forces and torques then come from lbdem_psc_forces
+ * on the same keys (_transfer_coarse_force :626-659; fine grains get zero there). */
+int lbdem_hybrid_map_coarse(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                            const lbdem_units *u, uint32_t *key, float *snap, int32_t *bbox,
+                            int32_t clear_previous, void *stream);
+
+/* Fine-grain spreading, HybridLattice3D.map_fine_grains (:260-376): for each grain with
+ * 2R <= dx, the mirror 3-point kernel over the active cells not fully covered by a coarse grain
+ * (key), normalised, corrected by the free fraction 1 - eps_c, spread into fine_volfrac,
+ * velsum and wsum (FP32 atomics); then per cell the clamp to the free fraction (status[0] +=
+ * cells that overflowed by more than 1e-6, the reference's warning) and the solid velocity
+ * velsum / fine_volfrac where fine_volfrac > 1e-10 (elsewhere fine_velsolid keeps its value). */
+int lbdem_hybrid_map_fine(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                          const lbdem_units *u, const uint32_t *key, float *fine_volfrac, float *velsum,
+                          float *wsum, float *fine_velsolid, int32_t *status, void *stream);
+
+/* Tenneti drag on the fine grains, HybridLattice3D._transfer_fine_force and
+ * _compute_drag_force (:661-740): kernel-interpolated fluid velocity (vel: the post-stream
+ * velocity field) and the total solid fraction fine + coarse.  Writes g->force_fluid of the
+ * grains with 2R <= dx only; call it after lbdem_psc_forces, which writes every grain. */
+int lbdem_hybrid_fine_drag(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                           const lbdem_units *u, const uint32_t *key, const float *fine_volfrac,
+                           const float *vel, void *stream);
+
+/* Tenneti weight of the fine grains per cell, HybridLattice3D.compute_fine_weights (:378-431),
+ * for inspection (the collision evaluates it inline); cells where it is not defined keep the
+ * weight they hold, as in the reference. */
+int lbdem_hybrid_fine_weight(const lbdem_lattice *lat, const uint8_t *flags, const uint32_t *key,
+                             const float *fine_volfrac, const float *fine_velsolid, const float *vel,
+                             float *weight, void *stream);
 
 /* ------------------------------------------------------------------------------------ */
 /* unresolved coupling (src/unlbdem)                                                      */

@@ -26,7 +26,8 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-CONTRACT = {"lbm_f32.hip", "lbm_f16.hip", "lbm_bf16.hip", "lbm_f64.hip", "adjoint.hip"}
+CONTRACT = {"lbm_f32.hip", "lbm_f16.hip", "lbm_bf16.hip", "lbm_f64.hip", "lbm_hyb_f32.hip", "lbm_hyb_f16.hip",
+            "lbm_hyb_bf16.hip", "lbm_hyb_f64.hip", "adjoint.hip"}
 
 
 def _compile(src):
@@ -42,7 +43,9 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    # largest first: the collide-stream translation units take the longest
+    srcs = sorted((os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip")),
+                  key=lambda f: (not os.path.basename(f).startswith("lbm_"), f))
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, srcs))

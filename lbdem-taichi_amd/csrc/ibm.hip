@@ -22,6 +22,7 @@ struct IbmArgs {
   float *volfrac, *velsum, *wsum, *velsolid, *fvel, *frho, *hforce;
   double *force_out;
   int32_t *status;
+  const uint32_t *key;  // hybrid coupling: coarse-grain keys of the PSC mapping
 };
 
 // EqIMBlattice3D.threedelta (src/unlbdem/eqlattice.py:451-462)
@@ -220,6 +221,166 @@ __global__ __launch_bounds__(256) void k_eq_weight(const IbmArgs a, const KParam
   if (!(kp.flags[c] & LBDEM_FLAG_ACTIVE)) return;
   const float e = a.volfrac[c];
   if (!(e > 0.f)) return;
+  float d2 = 0.f;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const float d = a.velsolid[3 * c + m] - a.fvel[3 * c + m];
+    d2 += d * d;
+  }
+  weight[c] = tenneti_weight<float>(e, sqrtf(d2), (float)a.nu_lu);
+}
+
+// ------------------------------------------------------------------------------------
+// Hybrid coupling, fine grains (HybridLattice3D, src/hybridlbdem/hybridlattice.py)
+// ------------------------------------------------------------------------------------
+// coarse solid fraction of a cell from its PSC key: 1 (full cover) or count / 125
+__device__ __forceinline__ double coarse_eps(uint32_t key) {
+  if (key == 0u) return 0.0;
+  if (key & (1u << 30)) return 1.0;
+  return (double)((key >> 23) & 0x7Fu) / 125.0;
+}
+
+// map_fine_grains passes 1-3 (hybridlattice.py:284-361), one thread per grain with 2R <= dx:
+// kernel sum, free-fraction correction sum, then the spread (FP32 atomics)
+__global__ __launch_bounds__(256) void k_hyb_spread(const IbmArgs a) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.np) return;
+  const double R = a.rad[p];
+  if (2.0 * R > a.dx) return;  // coarse grain (PSC side)
+  double xc, yc, zc;
+  lattice_pos(a, p, xc, yc, zc);
+  int i0, i1, j0, j1, k0, k1;
+  eq_box(xc, a.nx, i0, i1);
+  eq_box(yc, a.ny, j0, j1);
+  eq_box(zc, a.nz, k0, k1);
+  // the cells a fine grain may occupy: active and not fully covered by a coarse grain
+  auto open_cell = [&](int c, double &ec) {
+    if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) return false;
+    ec = coarse_eps(a.key[c]);
+    return ec < 1.0;
+  };
+  double denom = 0.0;
+  for (int i = i0; i < i1; ++i)
+    for (int j = j0; j < j1; ++j)
+      for (int k = k0; k < k1; ++k) {
+        double ec;
+        if (!open_cell((i * a.ny + j) * a.nz + k, ec)) continue;
+        const double wb = kernel_mirror(a, xc, yc, zc, i, j, k);
+        if (wb > 0.0) denom += wb;
+      }
+  if (denom < 1e-30) return;
+  double dcorr = 0.0;
+  for (int i = i0; i < i1; ++i)
+    for (int j = j0; j < j1; ++j)
+      for (int k = k0; k < k1; ++k) {
+        double ec;
+        if (!open_cell((i * a.ny + j) * a.nz + k, ec)) continue;
+        const double wb = kernel_mirror(a, xc, yc, zc, i, j, k);
+        if (wb > 0.0) dcorr += (wb / denom) * (1.0 - ec);
+      }
+  if (dcorr < 1e-30) return;
+  const double pi = 3.14159265358979323846;
+  const double Vp = 4.0 / 3.0 * pi * R * R * R, Vl = a.dx * a.dx * a.dx;
+  const double vx = a.vel[3 * p] * a.dt / a.dx, vy = a.vel[3 * p + 1] * a.dt / a.dx,
+               vz = a.vel[3 * p + 2] * a.dt / a.dx;
+  for (int i = i0; i < i1; ++i)
+    for (int j = j0; j < j1; ++j)
+      for (int k = k0; k < k1; ++k) {
+        const int c = (i * a.ny + j) * a.nz + k;
+        double ec;
+        if (!open_cell(c, ec)) continue;
+        const double wb = kernel_mirror(a, xc, yc, zc, i, j, k);
+        if (wb <= 0.0) continue;
+        const double wn = wb / denom;
+        const double e = wn * (1.0 - ec) / dcorr * Vp / Vl;
+        unsafeAtomicAdd(a.volfrac + c, (float)e);
+        unsafeAtomicAdd(a.velsum + 3 * c, (float)(vx * e));
+        unsafeAtomicAdd(a.velsum + 3 * c + 1, (float)(vy * e));
+        unsafeAtomicAdd(a.velsum + 3 * c + 2, (float)(vz * e));
+        unsafeAtomicAdd(a.wsum + c, (float)wn);
+      }
+}
+
+// map_fine_grains, last loop (hybridlattice.py:363-376): clamp to the free fraction, then
+// the solid velocity where the fraction exceeds 1e-10 (elsewhere it keeps its value)
+__global__ __launch_bounds__(256) void k_hyb_finalize(const IbmArgs a, int ncell) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncell) return;
+  const double avail = 1.0 - coarse_eps(a.key[c]);
+  float e = a.volfrac[c];
+  if ((double)e > avail) {
+    if ((double)e - avail > 1e-6 && a.status) atomicAdd(a.status, 1);
+    e = (float)fmax(0.0, avail - 0.01);
+    a.volfrac[c] = e;
+  }
+  if (e > 1e-10f) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) a.velsolid[3 * c + m] = a.velsum[3 * c + m] / e;
+  }
+}
+
+// _transfer_fine_force + _compute_drag_force (hybridlattice.py:661-740), one thread per grain
+// with 2R <= dx; nodes with a positive kernel weight only; total solid fraction fine + coarse
+__global__ __launch_bounds__(256) void k_hyb_drag(const IbmArgs a) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= a.np) return;
+  if (2.0 * a.rad[p] > a.dx) return;
+  double xc, yc, zc;
+  lattice_pos(a, p, xc, yc, zc);
+  int i0, i1, j0, j1, k0, k1;
+  eq_box(xc, a.nx, i0, i1);
+  eq_box(yc, a.ny, j0, j1);
+  eq_box(zc, a.nz, k0, k1);
+  double vw[3] = {0, 0, 0}, es = 0.0, wt = 0.0;
+  int nl = 0;
+  for (int i = i0; i < i1; ++i)
+    for (int j = j0; j < j1; ++j)
+      for (int k = k0; k < k1; ++k) {
+        const int c = (i * a.ny + j) * a.nz + k;
+        if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) continue;
+        const double w = kernel_mirror(a, xc, yc, zc, i, j, k);
+        if (w <= 0.0) continue;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) vw[m] += (double)a.fvel[3 * c + m] * w;
+        es += 1.0 - (double)a.volfrac[c] - coarse_eps(a.key[c]);
+        wt += w;
+        nl += 1;
+      }
+  double uf[3] = {0, 0, 0}, ef = 0.0;
+  if (wt > 1e-15) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) uf[m] = vw[m] / wt * a.dx / a.dt;
+  }
+  if (nl > 0) ef = es / (double)nl;
+  double F[3] = {0, 0, 0};
+  const double ep = 1.0 - ef;
+  if (ep > 1e-15) {
+    const double pi = 3.14159265358979323846;
+    const double dp = 2.0 * a.rad[p];
+    const double mu = a.rho * a.nu_lu * a.dx * a.dx / a.dt;
+    double us[3], m2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      us[m] = a.vel[3 * p + m] - uf[m];
+      m2 += us[m] * us[m];
+    }
+    const double Re = (1.0 - ep) * a.rho * dp * sqrt(m2) / mu;
+    const double Cd = tenneti_cd(Re, ep, 1e-9);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[m] = -3.0 * pi * dp * mu * (1.0 - ep) * Cd * us[m];
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) a.force_out[3 * p + m] = F[m];
+}
+
+// compute_fine_weights (hybridlattice.py:378-398) for inspection
+__global__ __launch_bounds__(256) void k_hyb_weight(const IbmArgs a, const KParams kp, float *weight) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= kp.ncell) return;
+  if (!(kp.flags[c] & LBDEM_FLAG_ACTIVE)) return;
+  const float e = a.volfrac[c];
+  const double fvf = 1.0 - (double)e - coarse_eps(a.key[c]);
+  if (!(e > 0.f) || !(fvf > 1e-10)) return;
   float d2 = 0.f;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
@@ -468,6 +629,68 @@ int lbdem_vans_drag(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_
   a.force_out = g->force_fluid;
   if (g->n > 0) hipLaunchKernelGGL(k_vans_drag, dim3(nblk(g->n)), dim3(256), 0, as_stream(stream), a);
   LBDEM_CHECK_LAUNCH("lbdem_vans_drag");
+  return LBDEM_OK;
+}
+
+int lbdem_hybrid_map_fine(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                          const lbdem_units *u, const uint32_t *key, float *fine_volfrac, float *velsum,
+                          float *wsum, float *fine_velsolid, int32_t *status, void *stream) {
+  IbmArgs a;
+  int rc = fill_ibm(lat, flags, g, u, 0.0, a, "lbdem_hybrid_map_fine");
+  if (rc) return rc;
+  LBDEM_REQUIRE(key && fine_volfrac && velsum && wsum && fine_velsolid, "lbdem_hybrid_map_fine: null argument");
+  LBDEM_REQUIRE(g->n == 0 || (g->position && g->velocity && g->radius), "lbdem_hybrid_map_fine: grain arrays");
+  const int64_t n = (int64_t)lat->nx * lat->ny * lat->nz;
+  a.key = key;
+  a.volfrac = fine_volfrac;
+  a.velsum = velsum;
+  a.wsum = wsum;
+  a.velsolid = fine_velsolid;
+  a.status = status;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, fine_volfrac, (size_t)n);
+  hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, velsum, (size_t)(3 * n));
+  hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, wsum, (size_t)n);
+  if (g->n > 0) hipLaunchKernelGGL(k_hyb_spread, dim3(nblk(g->n)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_hyb_finalize, dim3(nblk(n)), dim3(256), 0, s, a, (int)n);
+  LBDEM_CHECK_LAUNCH("lbdem_hybrid_map_fine");
+  return LBDEM_OK;
+}
+
+int lbdem_hybrid_fine_drag(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                           const lbdem_units *u, const uint32_t *key, const float *fine_volfrac, const float *vel,
+                           void *stream) {
+  IbmArgs a;
+  int rc = fill_ibm(lat, flags, g, u, (1.0 / lat->omega0 - 0.5) / 3.0, a, "lbdem_hybrid_fine_drag");
+  if (rc) return rc;
+  LBDEM_REQUIRE(key && fine_volfrac && vel, "lbdem_hybrid_fine_drag: null argument");
+  LBDEM_REQUIRE(g->n == 0 || (g->position && g->velocity && g->radius && g->force_fluid),
+                "lbdem_hybrid_fine_drag: grain arrays");
+  a.key = key;
+  a.volfrac = const_cast<float *>(fine_volfrac);
+  a.fvel = const_cast<float *>(vel);
+  a.force_out = g->force_fluid;
+  if (g->n > 0) hipLaunchKernelGGL(k_hyb_drag, dim3(nblk(g->n)), dim3(256), 0, as_stream(stream), a);
+  LBDEM_CHECK_LAUNCH("lbdem_hybrid_fine_drag");
+  return LBDEM_OK;
+}
+
+int lbdem_hybrid_fine_weight(const lbdem_lattice *lat, const uint8_t *flags, const uint32_t *key,
+                             const float *fine_volfrac, const float *fine_velsolid, const float *vel, float *weight,
+                             void *stream) {
+  LBDEM_REQUIRE(lat && flags && key && fine_volfrac && fine_velsolid && vel && weight,
+                "lbdem_hybrid_fine_weight: null argument");
+  IbmArgs a{};
+  a.key = key;
+  a.volfrac = const_cast<float *>(fine_volfrac);
+  a.velsolid = const_cast<float *>(fine_velsolid);
+  a.fvel = const_cast<float *>(vel);
+  a.nu_lu = (1.0 / lat->omega0 - 0.5) / 3.0;
+  KParams kp{};
+  kp.ncell = lat->nx * lat->ny * lat->nz;
+  kp.flags = flags;
+  hipLaunchKernelGGL(k_hyb_weight, dim3(nblk(kp.ncell)), dim3(256), 0, as_stream(stream), a, kp, weight);
+  LBDEM_CHECK_LAUNCH("lbdem_hybrid_fine_weight");
   return LBDEM_OK;
 }
 

@@ -34,7 +34,7 @@ static int fill_params(const lbdem_lattice *lat, const lbdem_step_args *a, KPara
                 lat->ny, lat->nz);
   const int64_t n = (int64_t)lat->nx * lat->ny * lat->nz;
   LBDEM_REQUIRE(n < (int64_t(1) << 31), "%s: %lld cells exceed 2^31", who, (long long)n);
-  LBDEM_REQUIRE(lat->coupling >= 0 && lat->coupling < 4, "%s: bad coupling %d", who, lat->coupling);
+  LBDEM_REQUIRE(lat->coupling >= 0 && lat->coupling <= LBDEM_COUPLE_HYBRID, "%s: bad coupling %d", who, lat->coupling);
   LBDEM_REQUIRE(lat->collision >= 0 && lat->collision < 3, "%s: bad collision %d", who, lat->collision);
   LBDEM_REQUIRE(a->f && a->flags && a->nmask, "%s: f, flags and nmask are required", who);
   LBDEM_REQUIRE(lat->omega0 > 0.0, "%s: omega0 must be positive", who);
@@ -183,6 +183,9 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
                 "lbdem_lbm_step: PSC needs psc_key, psc_grain, psc_force");
   LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_EQIBM || (a->volfrac && a->velsolid && a->nu_lu > 0),
                 "lbdem_lbm_step: EqIBM needs volfrac, velsolid and nu_lu > 0");
+  LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_HYBRID ||
+                    (a->psc_key && a->psc_grain && a->psc_force && a->volfrac && a->velsolid && a->nu_lu > 0),
+                "lbdem_lbm_step: hybrid coupling needs psc_key, psc_grain, psc_force, volfrac, velsolid and nu_lu > 0");
   LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_VANS ||
                     (a->volfrac && a->hforce && a->volfrac_prev && a->hforce_prev && a->nu_lu > 0),
                 "lbdem_lbm_step: VANS needs volfrac, hforce, volfrac_prev, hforce_prev, nu_lu > 0");

@@ -283,8 +283,10 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
   const T usq = ux * ux + uy * uy + uz * uz;
   const T rf = rho * fvf;
 
-  // solid coupling weight B and solid velocity
+  // solid coupling weight B and solid velocity (hybrid: B and us of the coarse grain, beta and
+  // uf of the fine grains)
   T B = T(0), usx = T(0), usy = T(0), usz = T(0);
+  T beta = T(0), ufx = T(0), ufy = T(0), ufz = T(0);
   bool solid = false;
   if constexpr (COUP == LBDEM_COUPLE_PSC) {
     const uint32_t key = p.psc_key[c];
@@ -315,6 +317,39 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
       usz = (T)p.velsolid[3 * c + 2];
       const T dx = usx - ux, dy = usy - uy, dz = usz - uz;
       B = tenneti_weight<T>(eps, sqrt(dx * dx + dy * dy + dz * dz), T(p.nu_lu));
+    }
+  } else if constexpr (COUP == LBDEM_COUPLE_HYBRID) {
+    // HybridLattice3D.collide (src/hybridlbdem/hybridlattice.py:433-565): coarse grain from the
+    // key, B = eps (tau - 1/2) / ((1 - eps) + (tau - 1/2)) with this step's omega
+    // (_psc_compute_weight :236-244); fine grains from the fractions, Tenneti weight beta where
+    // the cell keeps a fluid fraction 1 - eps_f - eps_c > 1e-10 (compute_fine_weights
+    // :378-398); beta <= 1 - B where both are present (collide_hybrid :535-537).
+    const uint32_t key = p.psc_key[c];
+    T ec = T(0);
+    if (key != 0u) {
+      solid = true;
+      const int g = 0x7FFFFF - (int)(key & 0x7FFFFFu);
+      const float *G = p.psc_grain + 12 * g;
+      const T rx = T(i) - (T)G[0], ry = T(j) - (T)G[1], rz = T(k) - (T)G[2];
+      const T wx = (T)G[6], wy = (T)G[7], wz = (T)G[8];
+      usx = (T)G[3] + (wy * rz - wz * ry);
+      usy = (T)G[4] + (wz * rx - wx * rz);
+      usz = (T)G[5] + (wx * ry - wy * rx);
+      ec = (key & (1u << 30)) ? T(1) : T((key >> 23) & 0x7Fu) / T(125);
+      const T th = recip(omega) - T(0.5);
+      B = ec * th / ((T(1) - ec) + th);
+    }
+    const T ef = (T)p.volfrac[c];
+    if (ef > T(0)) {
+      solid = true;
+      ufx = (T)p.velsolid[3 * c];
+      ufy = (T)p.velsolid[3 * c + 1];
+      ufz = (T)p.velsolid[3 * c + 2];
+      if (T(1) - ef - ec > T(1e-10)) {
+        const T dx = ufx - ux, dy = ufy - uy, dz = ufz - uz;
+        beta = tenneti_weight<T>(ef, sqrt(dx * dx + dy * dy + dz * dz), T(p.nu_lu));
+      }
+      if (key != 0u && B + beta > T(1)) beta = T(1) - B;
     }
   }
 
@@ -363,15 +398,21 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
     // weighted collision fpc = f + B Omega_s + (1-B) Omega_f with
     // Omega_s = f_qbar - feq_qbar + feq_s,q - f_q,  Omega_f = -omega (f_q - feq_q)
     // (src/lbdem/psclattice3d.py:240-261, src/unlbdem/eqlattice.py:272-292)
+    // Hybrid: f + beta Omega_s1 + B Omega_s2 + (1 - B - beta) Omega_f, Omega_s1 with the fine
+    // solid velocity, Omega_s2 with the coarse one (collide_hybrid, hybridlattice.py:523-565;
+    // beta = 0 in coarse-only and B = 0 in fine-only cells gives collide_coarse / collide_fine)
     const T ussq = usx * usx + usy * usy + usz * usz;
+    const T ufsq = ufx * ufx + ufy * ufy + ufz * ufz;
     T hx = T(0), hy = T(0), hz = T(0);
-    const T oB = T(1) - B;
+    const T oB = COUP == LBDEM_COUPLE_HYBRID ? T(1) - B - beta : T(1) - B;
     {
       // rest population: its own opposite
       const T feq0 = T(kW[0]) * (dr - rho * T(1.5) * usq);
       const T feqs0 = T(kW[0]) * (dr - rho * T(1.5) * ussq);
       const T Os = feqs0 - feq0;
-      f[0] = f[0] + B * Os - oB * omega * (f[0] - feq0);
+      T n0 = f[0] + B * Os - oB * omega * (f[0] - feq0);
+      if constexpr (COUP == LBDEM_COUPLE_HYBRID) n0 += beta * (T(kW[0]) * (dr - rho * T(1.5) * ufsq) - feq0);
+      f[0] = n0;
     }
     static_for<1, Q>([&](auto qc) {
       constexpr int q = qc;
@@ -389,6 +430,13 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
         const T Os_r = fq - feq_q + fs_r - fr;
         T nq = fq + B * Os_q - oB * omega * (fq - feq_q);
         T nr = fr + B * Os_r - oB * omega * (fr - feq_r);
+        if constexpr (COUP == LBDEM_COUPLE_HYBRID) {
+          const T cuf = cdot<q>(ufx, ufy, ufz);
+          const T symf = dr + rho * (T(4.5) * cuf * cuf - T(1.5) * ufsq);
+          const T ff_q = w * (symf + T(3) * rho * cuf), ff_r = w * (symf - T(3) * rho * cuf);
+          nq += beta * (fr - feq_r + ff_q - fq);
+          nr += beta * (fq - feq_q + ff_r - fr);
+        }
         if (forced) {
           const T cF = cdot<q>(F3x, F3y, F3z);
           nq += oB * w * cF;
@@ -403,7 +451,7 @@ __device__ __forceinline__ T collide(const KParams &p, int c, int i, int j, int 
         acc<kCZ[q]>(hz, -d);
       }
     });
-    if constexpr (COUP == LBDEM_COUPLE_PSC) {
+    if constexpr (COUP == LBDEM_COUPLE_PSC || COUP == LBDEM_COUPLE_HYBRID) {
       p.psc_force[3 * c] = (float)hx;
       p.psc_force[3 * c + 1] = (float)hy;
       p.psc_force[3 * c + 2] = (float)hz;

@@ -8,12 +8,12 @@ using KernelFn = void (*)(const KParams);
 using BcKernelFn = void (*)(const KParams, const int32_t *, const uint8_t *, int, float *, float *);
 
 struct LbmKernels {
-  KernelFn step[3][4][2];  // [collision][coupling][odd]; nullptr = unsupported combination
-  KernelFn pencil[3][4][2];   // row-tiled variant of step, one cell per thread
-  KernelFn pencil2[3][4][2];  // row-tiled variant, two cells per thread
-  KernelFn pencil4[3][4][2];  // row-tiled variant, four cells per thread (not for FP64)
-  KernelFn init[4];        // [coupling]
-  KernelFn macros[4];      // [coupling]
+  KernelFn step[3][5][2];  // [collision][coupling][odd]; nullptr = unsupported combination
+  KernelFn pencil[3][5][2];   // row-tiled variant of step, one cell per thread
+  KernelFn pencil2[3][5][2];  // row-tiled variant, two cells per thread
+  KernelFn pencil4[3][5][2];  // row-tiled variant, four cells per thread (not for FP64)
+  KernelFn init[5];        // [coupling]
+  KernelFn macros[5];      // [coupling]
   KernelFn read_f;
   KernelFn write_f;
   BcKernelFn boundaries;
@@ -25,10 +25,17 @@ const LbmKernels &lbm_kernels_f16();
 const LbmKernels &lbm_kernels_bf16();
 const LbmKernels &lbm_kernels_f64();
 
+// hybrid-coupling entries, instantiated in their own translation units (lbm_hyb_<type>.hip)
+// so that the four storage types and the two coupling sets compile in parallel
+void add_hybrid_f32(LbmKernels &k);
+void add_hybrid_f16(LbmKernels &k);
+void add_hybrid_bf16(LbmKernels &k);
+void add_hybrid_f64(LbmKernels &k);
+
 }  // namespace lbdem
 
 // Builds the table for storage type S (expanded once in each lbm_<type>.hip).
-#define LBDEM_DEFINE_LBM_TABLE(S, NAME)                                                        \
+#define LBDEM_DEFINE_LBM_TABLE(S, NAME, ADD_HYBRID)                                            \
   namespace lbdem {                                                                             \
   const LbmKernels &NAME() {                                                                    \
     static const LbmKernels t = [] {                                                            \
@@ -105,6 +112,9 @@ const LbmKernels &lbm_kernels_f64();
       k.macros[LBDEM_COUPLE_PSC] = k_macros<S, LBDEM_COUPLE_NONE>;                              \
       k.macros[LBDEM_COUPLE_EQIBM] = k_macros<S, LBDEM_COUPLE_NONE>;                            \
       k.macros[LBDEM_COUPLE_VANS] = k_macros<S, LBDEM_COUPLE_VANS>;                             \
+      k.init[LBDEM_COUPLE_HYBRID] = k_init<S, LBDEM_COUPLE_NONE>;                               \
+      k.macros[LBDEM_COUPLE_HYBRID] = k_macros<S, LBDEM_COUPLE_NONE>;                           \
+      ADD_HYBRID(k);                                                                            \
       k.read_f = k_read_f<S>;                                                                   \
       k.write_f = k_write_f<S>;                                                                 \
       k.boundaries = k_boundaries<S>;                                                           \
@@ -112,5 +122,22 @@ const LbmKernels &lbm_kernels_f64();
       return k;                                                                                 \
     }();                                                                                        \
     return t;                                                                                   \
+  }                                                                                             \
+  }
+
+// Hybrid coupling (lagged and fixed-point Smagorinsky, as BasicLattice3D.computeOmega).
+#define LBDEM_HYB_ENTRIES(S, COLL, ODD)                                                          \
+  k.step[COLL][LBDEM_COUPLE_HYBRID][ODD] = k_step<S, COLL, LBDEM_COUPLE_HYBRID, ODD>;            \
+  k.pencil[COLL][LBDEM_COUPLE_HYBRID][ODD] = k_step_pencil<S, COLL, LBDEM_COUPLE_HYBRID, ODD, 1>; \
+  k.pencil2[COLL][LBDEM_COUPLE_HYBRID][ODD] = k_step_pencil<S, COLL, LBDEM_COUPLE_HYBRID, ODD, 2>; \
+  k.pencil4[COLL][LBDEM_COUPLE_HYBRID][ODD] = pencil_fn<S, COLL, LBDEM_COUPLE_HYBRID, ODD, 4>();
+
+#define LBDEM_DEFINE_HYBRID_ENTRIES(S, NAME)                                                     \
+  namespace lbdem {                                                                             \
+  void NAME(LbmKernels &k) {                                                                    \
+    LBDEM_HYB_ENTRIES(S, LBDEM_COLL_SMAG_LAGGED, false)                                         \
+    LBDEM_HYB_ENTRIES(S, LBDEM_COLL_SMAG_LAGGED, true)                                          \
+    LBDEM_HYB_ENTRIES(S, LBDEM_COLL_SMAG_FIXED, false)                                          \
+    LBDEM_HYB_ENTRIES(S, LBDEM_COLL_SMAG_FIXED, true)                                           \
   }                                                                                             \
   }

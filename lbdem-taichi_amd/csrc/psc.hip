@@ -30,6 +30,7 @@ struct PscArgs {
   double *force_fluid, *moment_fluid;
   uint32_t *stale_key;     // keys of an earlier mapping, zeroed by the force kernel (may be null)
   const int *stale_bbox;   // that mapping's boxes
+  int coarse_only;         // hybrid coupling: grains with 2R <= dx are not mapped (empty box)
 };
 
 // bounding box of lattice2grains (psclattice3d.py:326-337): [int(xc - r), int(xc + r + 2))
@@ -81,6 +82,8 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
   double xc, yc, zc;
   int b[6];
   grain_box(a, g, xc, yc, zc, b);
+  // map_coarse_grains skips the fine grains (src/hybridlbdem/hybridlattice.py:197-199)
+  if (a.coarse_only && 2.0 * a.rad[g] <= a.dx) b[1] = b[0];
   const double r = (a.rad[g] - 0.5 * a.dx) / a.dx;
   const double half_diag = 0.5 * sqrt(3.0);
   const double r2 = sqrt_lt_bound(r);
@@ -284,14 +287,16 @@ using namespace lbdem;
 
 extern "C" {
 
-int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g, const lbdem_units *u,
-                  uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous, void *stream) {
+static int psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g, const lbdem_units *u,
+                   uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous, void *stream, int coarse_only,
+                   const char *who) {
   PscArgs a;
-  int rc = fill_psc(lat, g, u, a, "lbdem_psc_map");
+  int rc = fill_psc(lat, g, u, a, who);
   if (rc) return rc;
-  LBDEM_REQUIRE(flags && key && snap && bbox, "lbdem_psc_map: flags, key, snap and bbox are required");
+  LBDEM_REQUIRE(flags && key && snap && bbox, "%s: flags, key, snap and bbox are required", who);
   LBDEM_REQUIRE(g->n == 0 || (g->position && g->velocity && g->omega && g->radius),
-                "lbdem_psc_map: grain position, velocity, omega and radius are required");
+                "%s: grain position, velocity, omega and radius are required", who);
+  a.coarse_only = coarse_only;
   a.flags = flags;
   a.key = key;
   a.snap = snap;
@@ -299,11 +304,22 @@ int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_gr
   if (g->n == 0) return LBDEM_OK;
   if (clear_previous) {
     hipLaunchKernelGGL(k_psc_clear, dim3(g->n, kSplit), dim3(256), 0, as_stream(stream), a);
-    LBDEM_CHECK_LAUNCH("lbdem_psc_map(clear)");
+    LBDEM_CHECK_LAUNCH(who);
   }
   hipLaunchKernelGGL(k_psc_map, dim3(g->n, kSplit), dim3(256), 0, as_stream(stream), a);
-  LBDEM_CHECK_LAUNCH("lbdem_psc_map");
+  LBDEM_CHECK_LAUNCH(who);
   return LBDEM_OK;
+}
+
+int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g, const lbdem_units *u,
+                  uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous, void *stream) {
+  return psc_map(lat, flags, g, u, key, snap, bbox, clear_previous, stream, 0, "lbdem_psc_map");
+}
+
+int lbdem_hybrid_map_coarse(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                            const lbdem_units *u, uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous,
+                            void *stream) {
+  return psc_map(lat, flags, g, u, key, snap, bbox, clear_previous, stream, 1, "lbdem_hybrid_map_coarse");
 }
 
 int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u, const uint32_t *key,

@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("LBDEM_HIP_LIB") or os.path.normpath(os.path.join(_HER
 STORE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
 STORE_DTYPE = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}
 COLL_BGK, COLL_SMAG_LAGGED, COLL_SMAG_FIXED = 0, 1, 2
-COUPLE_NONE, COUPLE_PSC, COUPLE_EQIBM, COUPLE_VANS = 0, 1, 2, 3
+COUPLE_NONE, COUPLE_PSC, COUPLE_EQIBM, COUPLE_VANS, COUPLE_HYBRID = 0, 1, 2, 3, 4
 FLAG_ACTIVE, FLAG_NEAR, FLAG_WALL, FLAG_MOVING, FLAG_FLUID, FLAG_INLET = 0x01, 0x02, 0x04, 0x08, 0x10, 0x20
 BC_VEL_ZOUHE, BC_PRE_ZOUHE, BC_EXIT, BC_LADD_INLET = 1, 2, 3, 4
 
@@ -28,7 +28,7 @@ class Lattice(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class StepArgs(ctypes.Structure):

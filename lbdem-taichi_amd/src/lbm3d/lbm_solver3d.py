@@ -283,8 +283,8 @@ class BasicLattice3D:
             self.omega.touch()
 
     def stream(self):
-        """Streaming is fused into collide(); no wet-node boundaries are supported, so
-        there is nothing left to do here (lbm_solver3d.py:197-250)."""
+        """Streaming and the wet-node boundary pass run in collide(); nothing is left to do
+        here (lbm_solver3d.py:197-250)."""
 
     def step(self, n=1):
         for _ in range(n):

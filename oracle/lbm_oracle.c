@@ -751,3 +751,252 @@ void SFX(lbo_vans_collide)(int nx, int ny, int nz, const int32_t *CT, double ome
     }
   }
 }
+
+/* ------------------------------------------------------------------------------------- */
+/* HybridLattice3D, src/hybridlbdem/hybridlattice.py: coarse grains (2R > dx) as PSC, fine
+ * grains (2R <= dx) as particle-equivalence IBM, one collision with both weights. */
+#define HYB_SKIP (CT_OBSTACLE | CT_VEL_LADD | CT_FREE_SLIP | CT_VEL_INLET_LADD)
+
+/* map_coarse_grains, hybridlattice.py:173-234 and _psc_set_velsolid :246-257 (cell loop over
+ * the grains in index order).  grains: [x, y, z, vx, vy, vz, wx, wy, wz, R] per grain. */
+void SFX(lbo_hybrid_map_coarse)(int nx, int ny, int nz, const int32_t *CT, int32_t ngrain, const double *grains,
+                                const double *dmin, double dx, double dt, int32_t *id, REAL *volfrac,
+                                REAL *velsolid) {
+  const long n = (long)nx * ny * nz;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long c = 0; c < n; ++c) {
+    id[c] = -1;
+    volfrac[c] = 0;
+    velsolid[3 * c] = velsolid[3 * c + 1] = velsolid[3 * c + 2] = 0;
+    if (CT[c] & HYB_SKIP) continue;
+    const int i = (int)(c / ((long)ny * nz)), j = (int)((c / nz) % ny), k = (int)(c % nz);
+    for (int g = 0; g < ngrain; ++g) {
+      const double *G = grains + 10 * g;
+      if (2.0 * G[9] <= dx) continue;
+      const double xc = (G[0] - dmin[0] + 0.5 * dx) / dx, yc = (G[1] - dmin[1] + 0.5 * dx) / dx,
+                   zc = (G[2] - dmin[2] + 0.5 * dx) / dx;
+      const double r = (G[9] - 0.5 * dx) / dx;
+      const double dist = sqrt((xc - i) * (xc - i) + (yc - j) * (yc - j) + (zc - k) * (zc - k));
+      const double hd = 0.5 * sqrt(3.0);
+      int full = 0;
+      double eps = 1.0;
+      if (dist >= r + hd) continue;
+      if (dist <= r - hd) {
+        full = 1;
+      } else {
+        int cnt = 0;
+        for (int si = 0; si < 5; ++si)
+          for (int sj = 0; sj < 5; ++sj)
+            for (int sk = 0; sk < 5; ++sk) {
+              const double sx = i - 0.4 + 0.2 * si, sy = j - 0.4 + 0.2 * sj, sz = k - 0.4 + 0.2 * sk;
+              if (sqrt((xc - sx) * (xc - sx) + (yc - sy) * (yc - sy) + (zc - sz) * (zc - sz)) < r) ++cnt;
+            }
+        eps = cnt / 125.0;
+        if (!(eps > volfrac[c])) continue;
+      }
+      id[c] = g;
+      volfrac[c] = (REAL)eps;
+      const double rv[3] = {(i - xc) * dx, (j - yc) * dx, (k - zc) * dx};
+      const double oc[3] = {G[7] * rv[2] - G[8] * rv[1], G[8] * rv[0] - G[6] * rv[2], G[6] * rv[1] - G[7] * rv[0]};
+      for (int a = 0; a < 3; ++a) velsolid[3 * c + a] = (REAL)((G[3 + a] + oc[a]) * dt / dx);
+      if (full) break;
+    }
+  }
+}
+
+/* map_fine_grains, hybridlattice.py:260-376 (serial in grain order): three passes per fine
+ * grain (kernel sum, free-fraction correction, spread), then the clamp to the free fraction
+ * and the solid velocity.  grains: [x, y, z, vx, vy, vz, R].  fine_velsolid keeps its value
+ * where the fraction is at most 1e-10; *nwarn counts the overflow warnings. */
+void SFX(lbo_hybrid_map_fine)(int nx, int ny, int nz, const int32_t *CT, int32_t np, const double *grains,
+                              const double *dmin, double dx, double dt, const REAL *coarse_volfrac,
+                              REAL *fine_volfrac, REAL *fine_velsolid, REAL *velsum, REAL *wsum, int32_t *nwarn) {
+  const long n = (long)nx * ny * nz;
+  memset(fine_volfrac, 0, sizeof(REAL) * n);
+  memset(velsum, 0, sizeof(REAL) * 3 * n);
+  memset(wsum, 0, sizeof(REAL) * n);
+  const double Vl = dx * dx * dx, s = 1.5, pi = 3.14159265358979323846;
+  for (int p = 0; p < np; ++p) {
+    const double *G = grains + 7 * p;
+    if (2.0 * G[6] > dx) continue;
+    const double xc = (G[0] - dmin[0] + 0.5 * dx) / dx, yc = (G[1] - dmin[1] + 0.5 * dx) / dx,
+                 zc = (G[2] - dmin[2] + 0.5 * dx) / dx;
+    const double Vp = 4.0 / 3.0 * pi * G[6] * G[6] * G[6];
+    const double v[3] = {G[3] * dt / dx, G[4] * dt / dx, G[5] * dt / dx};
+    int i0 = (int)(xc - s), i1 = (int)(xc + s + 1), j0 = (int)(yc - s), j1 = (int)(yc + s + 1);
+    int k0 = (int)(zc - s), k1 = (int)(zc + s + 1);
+    if (i0 < 0) i0 = 0;
+    if (j0 < 0) j0 = 0;
+    if (k0 < 0) k0 = 0;
+    if (i1 > nx) i1 = nx;
+    if (j1 > ny) j1 = ny;
+    if (k1 > nz) k1 = nz;
+    double denom = 0.0, dcorr = 0.0;
+    for (int pass = 0; pass < 3; ++pass) {
+      for (int i = i0; i < i1; ++i)
+        for (int j = j0; j < j1; ++j)
+          for (int k = k0; k < k1; ++k) {
+            const long c = IDX(i, j, k);
+            if (CT[c] & HYB_SKIP) continue;
+            if (coarse_volfrac[c] >= 1.0) continue;
+            const double wb = kernel_mirror(nx, ny, nz, xc, yc, zc, i, j, k);
+            if (!(wb > 0.0)) continue;
+            if (pass == 0) {
+              denom += wb;
+            } else if (pass == 1) {
+              dcorr += (wb / denom) * (1.0 - coarse_volfrac[c]);
+            } else {
+              const double wn = wb / denom, avail = 1.0 - coarse_volfrac[c];
+              const double e = wn * avail / dcorr * Vp / Vl;
+              fine_volfrac[c] += e;
+              for (int a = 0; a < 3; ++a) velsum[3 * c + a] += v[a] * e;
+              wsum[c] += wn;
+            }
+          }
+      if (pass == 0 && denom < 1e-30) break;
+      if (pass == 1 && dcorr < 1e-30) break;
+    }
+  }
+  for (long c = 0; c < n; ++c) {
+    const double avail = 1.0 - coarse_volfrac[c];
+    if (fine_volfrac[c] > avail) {
+      if (fine_volfrac[c] - avail > 1e-6) ++*nwarn;
+      fine_volfrac[c] = (REAL)fmax(0.0, avail - 0.01);
+    }
+    if (fine_volfrac[c] > 1e-10)
+      for (int a = 0; a < 3; ++a) fine_velsolid[3 * c + a] = velsum[3 * c + a] / fine_volfrac[c];
+  }
+}
+
+/* compute_fine_weights + _weight_coefficient, hybridlattice.py:378-407 (physical units as
+ * written); cells where the weight is not defined keep their value */
+void SFX(lbo_hybrid_fine_weights)(int nx, int ny, int nz, const int32_t *CT, double dx, double dt,
+                                  double rho_phys, double omega0, const REAL *coarse_volfrac,
+                                  const REAL *fine_volfrac, const REAL *fine_velsolid, const REAL *vel,
+                                  REAL *fine_weight) {
+  const long n = (long)nx * ny * nz;
+  const double nuLu = (1.0 / omega0 - 0.5) / 3.0, nu = nuLu * dx * dx / dt, mu = rho_phys * nu;
+  const double pi = 3.14159265358979323846;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < n; ++c) {
+    if (CT[c] & HYB_SKIP) continue;
+    const double svf = fine_volfrac[c], fvf = 1.0 - svf - coarse_volfrac[c];
+    if (!(svf > 0.0 && fvf > 1e-10)) continue;
+    const double Reff = pow(3.0 * dx * dx * dx * svf / (4.0 * pi), 1.0 / 3.0), dp = 2.0 * Reff;
+    double s2 = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double d = (fine_velsolid[3 * c + a] - vel[3 * c + a]) * dx / dt;
+      s2 += d * d;
+    }
+    const double Re = (1.0 - svf) * rho_phys * dp * sqrt(s2) / mu;
+    const double Cd = tenneti_cd(Re, svf, 1e-9);
+    fine_weight[c] = (REAL)(3.0 * pi * (dp / dx) * nuLu * (1.0 - svf) * Cd);
+  }
+}
+
+/* HybridLattice3D.collide, hybridlattice.py:433-593: computeOmega (lagged Smagorinsky of
+ * BasicLattice3D), compute_feq, then the four-way dispatch collide_fluid / _coarse / _fine /
+ * _hybrid; coarse_weight = B of _psc_compute_weight (:236-244) at coarse cells. */
+void SFX(lbo_collide_hybrid)(int nx, int ny, int nz, const int32_t *CT, double omega0, double cs, const REAL *f,
+                             REAL *feq, const REAL *rho, const REAL *vel, REAL *omega, REAL *rate,
+                             const REAL *coarse_volfrac, const REAL *coarse_velsolid, REAL *coarse_weight,
+                             const REAL *fine_volfrac, const REAL *fine_weight, const REAL *fine_velsolid,
+                             REAL *hydroforce, REAL *fpc) {
+  const long n = (long)nx * ny * nz;
+  memset(hydroforce, 0, sizeof(REAL) * 3 * n);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < n; ++c) {
+    if (CT[c] & HYB_SKIP) continue;
+    smagorinsky(f + Q * c, feq + Q * c, rho[c], omega[c], omega0, cs, &omega[c], &rate[c]);
+    feq_of(rho[c], vel + 3 * c, feq + Q * c);
+    const REAL om = omega[c];
+    const REAL *fc = f + Q * c, *fe = feq + Q * c;
+    const int has_c = coarse_volfrac[c] > 0.0, has_f = fine_volfrac[c] > 0.0;
+    if (!has_c && !has_f) {
+      for (int q = 0; q < Q; ++q) fpc[Q * c + q] = (1.0 - om) * fc[q] + om * fe[q];
+      continue;
+    }
+    REAL B = 0, beta = 0, fs1[Q], fs2[Q];
+    if (has_c) {
+      const REAL eps = coarse_volfrac[c], tm = 1.0 / om - 0.5;
+      B = (eps * tm) / ((1.0 - eps) + tm);
+      coarse_weight[c] = B;
+      feq_of(rho[c], coarse_velsolid + 3 * c, fs2);
+    }
+    if (has_f) {
+      beta = fine_weight[c];
+      feq_of(rho[c], fine_velsolid + 3 * c, fs1);
+    }
+    if (has_c && has_f && B + beta > 1.0) beta = 1.0 - B;
+    for (int q = 0; q < Q; ++q) {
+      const REAL Of = -om * (fc[q] - fe[q]);
+      REAL v = fc[q];
+      if (has_f) v += beta * (fc[QINV[q]] - fe[QINV[q]] + fs1[q] - fc[q]);
+      if (has_c) {
+        const REAL Os2 = fc[QINV[q]] - fe[QINV[q]] + fs2[q] - fc[q];
+        v += B * Os2;
+        hydroforce[3 * c] -= B * Os2 * CX[q];
+        hydroforce[3 * c + 1] -= B * Os2 * CY[q];
+        hydroforce[3 * c + 2] -= B * Os2 * CZ[q];
+      }
+      v += (1.0 - B - beta) * Of;
+      fpc[Q * c + q] = v;
+    }
+  }
+}
+
+/* _transfer_fine_force + _compute_drag_force, hybridlattice.py:661-740; out: force_fluid of
+ * the fine grains (rows of the coarse grains untouched).  grains: [x, y, z, vx, vy, vz, R]. */
+void SFX(lbo_hybrid_fine_drag)(int nx, int ny, int nz, const int32_t *CT, int32_t np, const double *grains,
+                               const double *dmin, double dx, double dt, double rho_phys, double omega0,
+                               const REAL *vel, const REAL *fine_volfrac, const REAL *coarse_volfrac, double *out) {
+  const double nuLu = (1.0 / omega0 - 0.5) / 3.0, nu = nuLu * dx * dx / dt, mu = rho_phys * nu;
+  const double s = 1.5, pi = 3.14159265358979323846;
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < np; ++p) {
+    const double *G = grains + 7 * p;
+    if (2.0 * G[6] > dx) continue;
+    const double xc = (G[0] - dmin[0] + 0.5 * dx) / dx, yc = (G[1] - dmin[1] + 0.5 * dx) / dx,
+                 zc = (G[2] - dmin[2] + 0.5 * dx) / dx;
+    int i0 = (int)(xc - s), i1 = (int)(xc + s + 1), j0 = (int)(yc - s), j1 = (int)(yc + s + 1);
+    int k0 = (int)(zc - s), k1 = (int)(zc + s + 1);
+    if (i0 < 0) i0 = 0;
+    if (j0 < 0) j0 = 0;
+    if (k0 < 0) k0 = 0;
+    if (i1 > nx) i1 = nx;
+    if (j1 > ny) j1 = ny;
+    if (k1 > nz) k1 = nz;
+    double vw[3] = {0, 0, 0}, es = 0, wt = 0;
+    int nl = 0;
+    for (int i = i0; i < i1; ++i)
+      for (int j = j0; j < j1; ++j)
+        for (int k = k0; k < k1; ++k) {
+          const long c = IDX(i, j, k);
+          if (CT[c] & HYB_SKIP) continue;
+          const double w = kernel_mirror(nx, ny, nz, xc, yc, zc, i, j, k);
+          if (w <= 0.0) continue;
+          for (int a = 0; a < 3; ++a) vw[a] += vel[3 * c + a] * w;
+          es += (1.0 - fine_volfrac[c] - coarse_volfrac[c]);
+          wt += w;
+          nl += 1;
+        }
+    double uf[3] = {0, 0, 0}, ef = 0.0;
+    if (wt > 1e-15)
+      for (int a = 0; a < 3; ++a) uf[a] = vw[a] / wt * dx / dt;
+    if (nl > 0) ef = es / (double)nl;
+    const double ep = 1.0 - ef;
+    double F[3] = {0, 0, 0};
+    if (ep > 1e-15) {
+      const double dp = 2.0 * G[6];
+      double us[3], m2 = 0;
+      for (int a = 0; a < 3; ++a) {
+        us[a] = G[3 + a] - uf[a];
+        m2 += us[a] * us[a];
+      }
+      const double Re = (1.0 - ep) * rho_phys * dp * sqrt(m2) / mu;
+      const double Cd = tenneti_cd(Re, ep, 1e-9);
+      for (int a = 0; a < 3; ++a) F[a] = -3.0 * pi * dp * mu * (1.0 - ep) * Cd * us[a];
+    }
+    for (int a = 0; a < 3; ++a) out[3 * p + a] = F[a];
+  }
+}

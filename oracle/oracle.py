@@ -65,7 +65,9 @@ class OracleLattice:
     """Reference-layout lattice state and its two-pass step (collide -> fpc, stream -> f).
 
     kind: "basic" (lagged Smagorinsky, BasicLattice3D), "forced" (ForcedLattice3D),
-    "psc" (PSCLattice3D weighted collision), "eqibm" (EqIMBlattice3D), "vans".
+    "psc" (PSCLattice3D weighted collision), "eqibm" (EqIMBlattice3D), "vans",
+    "hybrid" (HybridLattice3D: id / volfrac / velsolid / weight hold the coarse fields,
+    fine_volfrac / fine_velsolid / fine_weight the fine ones).
     """
 
     def __init__(self, nx, ny, nz, omega, kind="basic", cs=0.1, F=None, real=np.float64):
@@ -91,10 +93,14 @@ class OracleLattice:
         self.velsolid = np.zeros(sh + (3,), dtype=real)
         self.hydroforce = np.zeros(sh + (3,), dtype=real)
         self.id = np.full(sh, -1, dtype=np.int32)
+        if kind == "hybrid":
+            self.fine_volfrac = np.zeros(sh, dtype=real)
+            self.fine_weight = np.zeros(sh, dtype=real)
+            self.fine_velsolid = np.zeros(sh + (3,), dtype=real)
 
     @property
     def skip(self):
-        return SKIP_BASIC if self.kind in ("basic", "eqibm") else SKIP_FORCED
+        return SKIP_BASIC if self.kind in ("basic", "eqibm", "hybrid") else SKIP_FORCED
 
     def _g(self):
         return (ctypes.c_int(self.nx), ctypes.c_int(self.ny), ctypes.c_int(self.nz))
@@ -119,6 +125,12 @@ class OracleLattice:
                                                    P(self.omega), P(self.rate), P(self.volfrac), P(self.weight),
                                                    P(self.velsolid),
                                                    P(self.hydroforce) if self.kind == "psc" else None, P(self.fpc))
+        elif self.kind == "hybrid":
+            _fn("lbo_collide_hybrid", self.real)(*self._g(), P(self.CT), _d(self.omega0), _d(self.cs), P(self.f),
+                                                 P(self.feq), P(self.rho), P(self.vel), P(self.omega), P(self.rate),
+                                                 P(self.volfrac), P(self.velsolid), P(self.weight),
+                                                 P(self.fine_volfrac), P(self.fine_weight), P(self.fine_velsolid),
+                                                 P(self.hydroforce), P(self.fpc))
         elif self.kind == "vans":
             _fn("lbo_vans_collide", self.real)(*self._g(), P(self.CT), R(self.omega0), P(self.f), P(self.feq),
                                                P(self.rho), P(self.vel), P(self.omega), P(self.volfrac),
@@ -225,6 +237,82 @@ def vans_drag(L, grains, dmin, dx, dt, rho_phys):
     _fn("lbo_vans_drag", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
                                  _d(rho_phys), _d(L.omega0), P(L.volfrac), P(L.vel), P(out))
     return out
+
+
+def hybrid_map_coarse(L, grains, dmin, dx, dt):
+    """HybridLattice3D.map_coarse_grains (hybridlattice.py:173-257); grains (n, 10) [x, v, w, R]."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    _fn("lbo_hybrid_map_coarse", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                                         P(L.id), P(L.volfrac), P(L.velsolid))
+    L.weight[...] = 0  # coarse_weight.fill(0.0), :193
+
+
+def hybrid_map_fine(L, grains, dmin, dx, dt):
+    """HybridLattice3D.map_fine_grains (hybridlattice.py:260-376); grains (n, 7) [x, v, R].
+    Returns (velsum, wsum, warnings)."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    velsum = np.zeros_like(L.fine_velsolid)
+    wsum = np.zeros_like(L.fine_volfrac)
+    nwarn = np.zeros(1, dtype=np.int32)
+    _fn("lbo_hybrid_map_fine", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g), P(dm), _d(dx), _d(dt),
+                                       P(L.volfrac), P(L.fine_volfrac), P(L.fine_velsolid), P(velsum), P(wsum),
+                                       P(nwarn))
+    return velsum, wsum, int(nwarn[0])
+
+
+def hybrid_fine_weights(L, dx, dt, rho_phys):
+    """HybridLattice3D.compute_fine_weights (hybridlattice.py:378-431)."""
+    _fn("lbo_hybrid_fine_weights", L.real)(*L._g(), P(L.CT), _d(dx), _d(dt), _d(rho_phys), _d(L.omega0),
+                                           P(L.volfrac), P(L.fine_volfrac), P(L.fine_velsolid), P(L.vel),
+                                           P(L.fine_weight))
+
+
+def hybrid_forces(L, grains, dmin, dx, dt, rho_phys):
+    """HybridLattice3D.lattice2grains (hybridlattice.py:595-740) -> (n, 6) [F, T]: the PSC
+    exchange of the coarse grains (_transfer_coarse_force), the Tenneti drag of the fine ones."""
+    g = np.ascontiguousarray(grains, dtype=np.float64)
+    out = psc_forces(L, g, dmin, dx, dt, rho_phys)
+    g7 = np.ascontiguousarray(np.concatenate([g[:, 0:6], g[:, 9:10]], axis=1))
+    dm = np.ascontiguousarray(dmin, dtype=np.float64)
+    drag = np.zeros((len(g), 3))
+    _fn("lbo_hybrid_fine_drag", L.real)(*L._g(), P(L.CT), ctypes.c_int32(len(g)), P(g7), P(dm), _d(dx), _d(dt),
+                                        _d(rho_phys), _d(L.omega0), P(L.vel), P(L.fine_volfrac), P(L.volfrac),
+                                        P(drag))
+    fine = 2.0 * g[:, 9] <= dx
+    out[fine, :3] = drag[fine]
+    out[fine, 3:] = 0.0
+    return out
+
+
+class HybridOracle:
+    """HybridLattice3D driven as experiment.py's main loop (hybridexperiment/experiment.py:
+    initialize_complete, then per step prepare_step, collide, stream, lattice2grains).
+    grains (n, 10) [x, v, omega, R], physical units; held fixed (no DEM)."""
+
+    def __init__(self, L, grains, dmin, dx, dt, rho_phys):
+        assert L.kind == "hybrid"
+        self.L, self.g = L, np.ascontiguousarray(grains, dtype=np.float64)
+        self.dmin, self.dx, self.dt, self.rho_phys = np.asarray(dmin, dtype=np.float64), dx, dt, rho_phys
+        self.warnings = 0
+
+    def prepare_step(self):
+        L, g = self.L, self.g
+        hybrid_map_coarse(L, g, self.dmin, self.dx, self.dt)
+        _, _, w = hybrid_map_fine(L, np.concatenate([g[:, 0:6], g[:, 9:10]], axis=1), self.dmin, self.dx, self.dt)
+        self.warnings += w
+        hybrid_fine_weights(L, self.dx, self.dt, self.rho_phys)
+
+    def initialize_complete(self):
+        self.L.initialize()
+        self.prepare_step()
+
+    def step(self):
+        self.prepare_step()
+        self.L.collide()
+        self.L.stream()
+        return hybrid_forces(self.L, self.g, self.dmin, self.dx, self.dt, self.rho_phys)
 
 
 def cavity(L, lid_u, lid_axis=2):

@@ -37,7 +37,7 @@ def test_python_binding_declares_every_header_function():
     H.lib()
     import importlib
     for mod in ("src.lbm3d", "src.lbdem.psclattice3d", "src.unlbdem.eqlattice", "src.unlbdem.vanslattice",
-                "src.dem3d.demsolver", "src.adjoint"):
+                "src.hybridlbdem.hybridlattice", "src.dem3d.demsolver", "src.adjoint"):
         try:
             importlib.import_module(mod)  # registers that module's prototypes
         except ModuleNotFoundError:
@@ -77,5 +77,5 @@ def test_header_constants_match_python_binding():
     for key, code in H.STORE.items():
         assert consts[f"LBDEM_STORE_{key.upper()}"] == code, key
     for name in ("COLL_BGK", "COLL_SMAG_LAGGED", "COLL_SMAG_FIXED", "COUPLE_NONE", "COUPLE_PSC", "COUPLE_EQIBM",
-                 "COUPLE_VANS"):
+                 "COUPLE_VANS", "COUPLE_HYBRID"):
         assert consts[f"LBDEM_{name}"] == getattr(H, name), name
