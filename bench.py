@@ -163,6 +163,106 @@ def channel_study(nx=256, ny=128, nz=128, steps=100):
             "peak_u_at_x16": round(float(ux.max()), 5)}
 
 
+def hybrid_study(steps=10, warmup=2, subcycles=200, cpu=True):
+    """The reference's hybrid experiment (example/hybridlbdem/hybridexperiment/experiment.py) at its
+    own shape: 28 x 28 x 148 lattice (dx 1.5 mm), the 37,464 grains of hybridcf.p4p (3 coarse of
+    R 19.75 mm, 37,461 fine of R 0.5-0.75 mm), all frozen; walls on the x and y faces, Ladd
+    velocity inlet (0.011 m/s) at z = 0, convective exit at z = Nz - 1; tau 0.50026 with lagged
+    Smagorinsky; Hertz contacts; 200 DEM substeps per lattice step; FP32 populations.  Per-phase
+    times from HIP events: prepare_step (coarse keys, fine spreading), collide + stream (with the
+    wet-node pass), lattice2grains (coarse forces, fine drag), and the DEM substeps.  The CPU
+    sample is one coupling step (prepare_step, collide, stream, lattice2grains) of the oracle on
+    the same state, on all host threads."""
+    import numpy as np
+    import torch
+    from src.dem3d.demconfig import DEMSolverConfig, DomainBounds, HertzContactConfig
+    from src.dem3d.demsolver import DEMSolver
+    from src.hybridlbdem.hybridlattice import HybridLattice3D
+    from src.lbm3d.lbmutils import CellType
+    dx, rho, mu, dens, tau, umax = 0.0015, 1000.0, 1e-3, 2450.0, 0.50026, 0.011
+    Nx, Ny, Nz = int(0.04 / dx) + 2, int(0.04 / dx) + 2, int(0.220 / dx) + 2
+    nu, nuLU = mu / rho, (tau - 0.5) / 3.0
+    dt = dx ** 2 / (nu / nuLU)
+    dom = DomainBounds(xmin=0.0, xmax=(Nx - 2) * dx, ymin=0.0, ymax=(Ny - 2) * dx, zmin=0.0, zmax=(Nz - 2) * dx)
+    cm = HertzContactConfig(pp_restitution=0.9, pw_restitution=0.9, pp_friction=0.1545, pw_friction=0.1333)
+    cfg = DEMSolverConfig(domain=dom, dt=dt / subcycles, gravity=(0.0, 0.0, -9.81 * (dens - rho) / dens),
+                          contact_model=cm)
+    cfg.set_particle_properties(elastic_modulus=5e10, poisson_ratio=0.25, max_coordinate_number=128)
+    cfg.set_wall_properties(elastic_modulus=2e10, poisson_ratio=0.3)
+    dem = DEMSolver(cfg)
+    dem.init_particle_fields(os.path.join(ROOT, "tests", "golden", "hybridcf.p4p"), [dom.xmin, dom.ymin, dom.zmin],
+                             [dom.xmax, dom.ymax, dom.zmax])
+    dem.set_contact_model("hertz")
+    dem.gf.freeze.fill(1)
+    lat = HybridLattice3D(Nx, Ny, Nz, 1.0 / tau, dx, dt, rho, dem)
+    uw = umax * dt / dx
+    lat.initialize_complete()
+    ct = np.full((Nx, Ny, Nz), CellType.FLUID, dtype=np.int32)
+    ct[:, :, 0] = CellType.VEL_INLET_LADD | CellType.BACK
+    ct[:, :, -1] = CellType.VEL_EXIT | CellType.FRONT
+    ct[0], ct[-1], ct[:, 0], ct[:, -1] = (CellType.OBSTACLE,) * 4
+    vel = np.zeros((Nx, Ny, Nz, 3), np.float32)
+    vel[1:-1, 1:-1, 0, 2] = uw
+    lat.CT.from_numpy(ct)
+    lat.vel.from_numpy(vel)
+
+    phases = ("prepare_step", "collide_stream", "lattice2grains", "dem")
+    ev = {k: [] for k in phases}
+
+    def step(record):
+        m = [torch.cuda.Event(enable_timing=True) for _ in range(5)] if record else None
+        if record:
+            m[0].record()
+        lat.prepare_step()
+        if record:
+            m[1].record()
+        lat.collide()
+        lat.stream()
+        if record:
+            m[2].record()
+        lat.lattice2grains()
+        if record:
+            m[3].record()
+        dem.run_substeps(subcycles)
+        if record:
+            m[4].record()
+            for i, k in enumerate(phases):
+                ev[k].append((m[i], m[i + 1]))
+
+    for _ in range(warmup):
+        step(False)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step(True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    out = {"study": "hybrid", "lattice": [Nx, Ny, Nz], "grains": int(dem.n),
+           "coarse_grains": int((2 * dem.gf.radius.to_numpy() > dx).sum()), "dem_substeps": subcycles,
+           "ms_per_lattice_step": round(ms, 3)}
+    for k in phases:
+        out[k + "_ms"] = round(sum(a.elapsed_time(b) for a, b in ev[k]) / steps, 4)
+    out["coupling_ms"] = round(out["prepare_step_ms"] + out["collide_stream_ms"] + out["lattice2grains_ms"], 4)
+    out["dem_us_per_substep"] = round(1e3 * out["dem_ms"] / subcycles, 2)
+    out["clamp_warnings"] = lat.clamp_count()
+    if cpu:
+        from oracle import oracle as O
+        orc = O.OracleLattice(Nx, Ny, Nz, 1.0 / tau, kind="hybrid", cs=lat.SmagorinskyConstant)
+        orc.CT[...] = ct
+        orc.vel[...] = vel
+        g = np.column_stack([dem.gf.position.to_numpy(), dem.gf.velocity.to_numpy(), dem.gf.omega.to_numpy(),
+                             dem.gf.radius.to_numpy()])
+        H = O.HybridOracle(orc, g, [dom.xmin, dom.ymin, dom.zmin], dx, dt, rho)
+        orc.initialize()
+        t0 = time.perf_counter()
+        H.step()
+        out["cpu_coupling_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+        out["cpu_threads"] = os.cpu_count()
+    return out
+
+
 def tgv_variants_study(n=256):
     """Collide-stream GLUPS of each storage type and row-tiled variant on the 256^3 TGV."""
     cases = [("f32", "fixed", 2), ("f32", "fixed", 3), ("f32", "lagged", 2), ("bf16", "fixed", 4),
@@ -431,7 +531,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv", "channel"], default=None,
+    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv", "channel", "hybrid"],
+                    default=None,
                     help="run a study instead of the benchmark and print its JSON summary")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 512 MiB three times (reference kernel for HBM byte counters)")
@@ -457,6 +558,9 @@ def main():
         return
     if args.study == "channel":
         print(json.dumps(channel_study()), flush=True)
+        return
+    if args.study == "hybrid":
+        print(json.dumps(hybrid_study()), flush=True)
         return
     if args.calibrate_copy:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")

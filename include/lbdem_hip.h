@@ -239,13 +239,13 @@ int lbdem_psc_decode(const lbdem_lattice *lat, const uint32_t *key, const float 
 /* hybrid resolved / unresolved coupling (src/hybridlbdem)                                */
 /* ------------------------------------------------------------------------------------ */
 /* Coarse-grain mapping, HybridLattice3D.map_coarse_grains (src/hybridlbdem/hybridlattice.py:
- * 173-234): lbdem_psc_map over the grains with 2R > dx only (the others get empty boxes).  The
- * PSC weight B is not mapped: the hybrid collision evaluates it with the step's own omega
- * (_psc_compute_weight :236-244).  Coarse forces and torques then come from lbdem_psc_forces
- * on the same keys (_transfer_coarse_force :626-659; fine grains get zero there). */
+ * 173-234): lbdem_psc_map over the grains with 2R > dx only.  coarse (device, ncoarse
+ * entries): the indices of those grains, one workgroup column each; NULL walks every grain and
+ * skips the fine ones.  The PSC weight B is not mapped: the hybrid collision evaluates it with
+ * the step's own omega (_psc_compute_weight :236-244). */
 int lbdem_hybrid_map_coarse(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
-                            const lbdem_units *u, uint32_t *key, float *snap, int32_t *bbox,
-                            int32_t clear_previous, void *stream);
+                            const lbdem_units *u, const int32_t *coarse, int32_t ncoarse, uint32_t *key,
+                            float *snap, int32_t *bbox, int32_t clear_previous, void *stream);
 
 /* Fine-grain spreading, HybridLattice3D.map_fine_grains (:260-376): for each grain with
  * 2R <= dx, the mirror 3-point kernel over the active cells not fully covered by a coarse grain
@@ -257,13 +257,17 @@ int lbdem_hybrid_map_fine(const lbdem_lattice *lat, const uint8_t *flags, const 
                           const lbdem_units *u, const uint32_t *key, float *fine_volfrac, float *velsum,
                           float *wsum, float *fine_velsolid, int32_t *status, void *stream);
 
-/* Tenneti drag on the fine grains, HybridLattice3D._transfer_fine_force and
- * _compute_drag_force (:661-740): kernel-interpolated fluid velocity (vel: the post-stream
- * velocity field) and the total solid fraction fine + coarse.  Writes g->force_fluid of the
- * grains with 2R <= dx only; call it after lbdem_psc_forces, which writes every grain. */
-int lbdem_hybrid_fine_drag(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
-                           const lbdem_units *u, const uint32_t *key, const float *fine_volfrac,
-                           const float *vel, void *stream);
+/* Hydrodynamic loads of all grains, HybridLattice3D.lattice2grains (:595-740): for the coarse
+ * grains (the ncoarse indices in coarse, as for lbdem_hybrid_map_coarse) force and torque from
+ * the momentum exchange the collision wrote per cell (_transfer_coarse_force :626-659, FP64,
+ * fixed-order reduction, as lbdem_psc_forces, with its stale-key clearing); for the others the
+ * Tenneti drag (_transfer_fine_force, _compute_drag_force :661-740) from the kernel-interpolated
+ * fluid velocity (vel: the post-stream velocity field) and the total solid fraction
+ * fine + coarse, and zero torque. */
+int lbdem_hybrid_forces(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
+                        const lbdem_units *u, const int32_t *coarse, int32_t ncoarse, const uint32_t *key,
+                        const float *cell_force, uint32_t *stale_key, const int32_t *stale_bbox,
+                        const float *fine_volfrac, const float *vel, void *stream);
 
 /* Tenneti weight of the fine grains per cell, HybridLattice3D.compute_fine_weights (:378-431),
  * for inspection (the collision evaluates it inline); cells where it is not defined keep the
@@ -354,8 +358,10 @@ typedef struct lbdem_dem_work {
   int32_t *count, *offset, *current; /* T each */
   int32_t *tile_sum;                 /* ceil(T / 1024) */
   int32_t *hash, *pid;               /* n each */
-  int32_t *pairs;                    /* 2 * pair_cap candidate pairs (i < j); the small-system
-                                        path lists the new contacts of a substep here */
+  int32_t *pairs;                    /* 2 * pair_cap candidate pairs (i < j): all of them
+                                        (lbdem_dem_broadphase), the touching ones (general path of
+                                        lbdem_dem_run), or the new contacts of a substep
+                                        (single-workgroup path) */
   int64_t pair_cap;
   unsigned long long *npairs;        /* 1: candidate count (may exceed pair_cap) */
   int32_t *need_new;                 /* pair_cap */
@@ -376,7 +382,10 @@ int lbdem_dem_broadphase(const lbdem_dem_params *P, const double *pos, const lbd
  * clear_state, contact (broad phase + resolve :638-740), resolve_wall (:809-838),
  * apply_body_force (:437-448), update (:514-636), late_clear_state (:431-436).
  * path: 0 automatic, 1 single-workgroup path (n <= 2048, one launch for all substeps),
- * 2 general grid-wide path. */
+ * 2 general grid-wide path.  The general path lists only the touching candidate pairs in
+ * w->pairs (and clears the history slots of candidates that stopped touching), which resolves
+ * every candidate as the reference does while the list stays the size of the contact set;
+ * w->npairs counts them, pair_cap bounds them (overflow: status[4]). */
 int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbdem_dem_work *w,
                   int32_t nsub, int32_t path, void *stream);
 

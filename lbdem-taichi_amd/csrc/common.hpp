@@ -166,4 +166,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// per-grain PSC force / torque reduction over the grains of glist (all grains when null);
+// shared by lbdem_psc_forces and the hybrid coupling (psc.hip)
+int launch_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u, const int32_t *glist,
+                      int32_t nlist, const uint32_t *key, const float *cell_force, uint32_t *stale_key,
+                      const int32_t *stale_bbox, void *stream, const char *who);
+
 }  // namespace lbdem

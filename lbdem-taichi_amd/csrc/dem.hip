@@ -74,15 +74,46 @@ __device__ __forceinline__ void probe_cells(const lbdem_dem_params &P, const dou
       if (h[e] == h[k]) h[k] = -1;  // a later match compares against -1 only, which changes nothing
 }
 
+// atomicAdd(ptr + h, 1) for each active lane; lanes of the wave sharing h are combined into one
+// atomic, for up to four distinct values (a coarse grid puts most of a wave into one or two
+// buckets, whose counters would otherwise serialise the whole launch).  Returns the old value
+// plus the lane's rank among the lanes combined with it.  Call from converged control flow.
+__device__ __forceinline__ int32_t agg_add_one(int32_t *ptr, int32_t h, bool active) {
+  const int lane = threadIdx.x & 63;
+  int32_t res = 0;
+  bool pending = active;
+  for (int r = 0; r < 4; ++r) {
+    const unsigned long long m = __ballot(pending);
+    if (m == 0ull) break;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int32_t hl = __shfl(h, leader, 64);
+    const bool mine = pending && h == hl;
+    const unsigned long long grp = __ballot(mine);
+    int32_t old = 0;
+    if (lane == leader) old = atomicAdd(ptr + hl, (int32_t)__popcll(grp));
+    old = __shfl(old, leader, 64);
+    if (mine) {
+      res = old + (int32_t)__popcll(grp & ((1ull << lane) - 1ull));
+      pending = false;
+    }
+  }
+  if (pending) res = atomicAdd(ptr + h, 1);
+  return res;
+}
+
 __global__ __launch_bounds__(256) void k_bp_count(const lbdem_dem_params P, const double *pos, int32_t *hash,
                                                   int32_t *count, double *force, double *moment) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P.n) return;
-  int32_t c[3];
-  cell_of(P, pos + 3 * i, c);
-  const int32_t h = hash3(c[0], c[1], c[2], P.T);
-  hash[i] = h;
-  atomicAdd(count + h, 1);
+  const bool active = i < P.n;
+  int32_t h = 0;
+  if (active) {
+    int32_t c[3];
+    cell_of(P, pos + 3 * i, c);
+    h = hash3(c[0], c[1], c[2], P.T);
+    hash[i] = h;
+  }
+  agg_add_one(count, h, active);
+  if (!active) return;
   if (force) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) force[3 * i + a] = moment[3 * i + a] = 0.0;  // clear_state
@@ -143,9 +174,10 @@ __global__ __launch_bounds__(256) void k_scan_add(int32_t *out, const int32_t *t
 __global__ __launch_bounds__(256) void k_bp_scatter(const lbdem_dem_params P, const int32_t *hash,
                                                     const int32_t *offset, int32_t *current, int32_t *pid) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P.n) return;
-  const int32_t h = hash[i];
-  pid[offset[h] + atomicAdd(current + h, 1)] = i;
+  const bool active = i < P.n;
+  const int32_t h = active ? hash[i] : 0;
+  const int32_t slot = agg_add_one(current, h, active);
+  if (active) pid[offset[h] + slot] = i;
 }
 
 // probe: candidate pairs (i, pid > i); one atomic per wavefront reserves the output range
@@ -451,6 +483,84 @@ __device__ __forceinline__ void resolve_new(const lbdem_dem_params &P, const lbd
   periodic_delta(P, S.pos + 3 * i, S.pos + 3 * j, d);
   const double dist = norm3(d);
   eval_pair(P, S, i, j, off, d, dist, dist - S.radius[i] - S.radius[j]);
+}
+
+// Candidate pairs of the multi-kernel path, filtered: only touching pairs are listed (the
+// candidate rule is the full probe's: j > i in one of i's probed buckets).  A candidate that
+// does not touch changes nothing unless it holds a history slot, which k_dem_release clears;
+// so the outcome is that of resolving every candidate, while the list stays the size of the
+// contact set (a strongly polydisperse packing, whose cells are 4 r_max wide, has O(n^2)
+// candidates).  blockIdx.y splits each particle's candidates over gridDim.y lanes.
+template <int NP>
+__global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P, const double *pos,
+                                                        const double *rad, const int32_t *count,
+                                                        const int32_t *offset, const int32_t *pid, int32_t *pairs,
+                                                        int64_t cap, unsigned long long *npairs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  const int split = gridDim.y, part = blockIdx.y;
+  const double *xi = pos + 3 * i;
+  int32_t c[3];
+  cell_of(P, xi, c);
+  int32_t h[NP];
+  probe_cells<NP>(P, xi, c, h);
+  if constexpr (NP == 8) {
+    // ascending bucket order: lanes whose probe sets share buckets (a coarse grid) walk them in
+    // step, so their candidate loads coincide
+#pragma unroll
+    for (int a = 0; a < NP - 1; ++a)
+#pragma unroll
+      for (int b = 0; b < NP - 1 - a; ++b) {
+        const int32_t lo = min(h[b], h[b + 1]), hi = max(h[b], h[b + 1]);
+        h[b] = lo;
+        h[b + 1] = hi;
+      }
+  }
+  const double ri = rad[i];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    if (h[k] < 0) continue;
+    const int32_t b = offset[h[k]], e = b + count[h[k]];
+    for (int idx = b + part; idx < e; idx += split) {
+      const int32_t j = pid[idx];
+      if (j <= i || !touching(P, xi, ri, pos + 3 * j, rad[j])) continue;
+      const unsigned long long w = atomicAdd(npairs, 1ull);
+      if (w < (unsigned long long)cap) {
+        pairs[2 * w] = i;
+        pairs[2 * w + 1] = j;
+      }
+    }
+  }
+}
+
+// The history slots of candidate pairs that no longer touch are released, as resolve_existing
+// does for every candidate with gap >= 0 (demsolver.py:638-668); contacts whose partner left
+// i's probed buckets keep their slot, as in the reference.  Runs before the resolve kernels.
+template <int NP>
+__global__ __launch_bounds__(256) void k_dem_release(const lbdem_dem_params P, const lbdem_dem_state S,
+                                                     const int32_t *hash) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  const int K = P.K;
+  const int m = min(S.cfn[i], K);
+  if (m == 0) return;
+  const double *xi = S.pos + 3 * i;
+  int32_t c[3];
+  cell_of(P, xi, c);
+  int32_t h[NP];
+  probe_cells<NP>(P, xi, c, h);
+  const double ri = S.radius[i];
+  for (int s = 0; s < m; ++s) {
+    const int off = i * K + s;
+    if (!S.cf_active[off]) continue;
+    const int j = S.cf_j[off];
+    if (j <= i) continue;
+    const int32_t hj = hash[j];
+    bool cand = false;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) cand |= (h[k] == hj);
+    if (cand && !touching(P, xi, ri, S.pos + 3 * j, S.radius[j])) S.cf_active[off] = 0;
+  }
 }
 
 // particle i's six wall-contact flags (wcf_active) as a bit mask, and back (changed bits only)
@@ -1073,7 +1183,7 @@ static int check_params(const lbdem_dem_params *P, const char *who) {
 }
 
 static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lbdem_dem_work *w, double *force,
-                          double *moment, hipStream_t s, const char *who) {
+                          double *moment, hipStream_t s, const char *who, const double *rad_touch = nullptr) {
   const int n = P->n;
   const unsigned nb = (unsigned)((n + 255) / 256);
   const int ntiles = (P->T + 1023) / 1024;
@@ -1082,12 +1192,23 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, ntiles);
   hipLaunchKernelGGL(k_scan_add, dim3((P->T + 255) / 256), dim3(256), 0, s, w->offset, w->tile_sum, P->T);
   hipLaunchKernelGGL(k_bp_scatter, dim3(nb), dim3(256), 0, s, *P, w->hash, w->offset, w->current, w->pid);
-  if (P->probe27)
+  if (rad_touch) {
+    // touching candidates only; with few particles each one's candidates are split over more
+    // lanes so that the launch still fills the chip
+    const unsigned split = (unsigned)std::min(16, std::max(1, (int)((2048 + nb - 1) / nb)));
+    if (P->probe27)
+      hipLaunchKernelGGL(k_bp_probe_touch<27>, dim3(nb, split), dim3(256), 0, s, *P, pos, rad_touch, w->count,
+                         w->offset, w->pid, w->pairs, w->pair_cap, w->npairs);
+    else
+      hipLaunchKernelGGL(k_bp_probe_touch<8>, dim3(nb, split), dim3(256), 0, s, *P, pos, rad_touch, w->count,
+                         w->offset, w->pid, w->pairs, w->pair_cap, w->npairs);
+  } else if (P->probe27) {
     hipLaunchKernelGGL(k_bp_probe<27>, dim3(nb), dim3(256), 0, s, *P, pos, w->count, w->offset, w->pid, w->pairs,
                        w->pair_cap, w->npairs);
-  else
+  } else {
     hipLaunchKernelGGL(k_bp_probe<8>, dim3(nb), dim3(256), 0, s, *P, pos, w->count, w->offset, w->pid, w->pairs,
                        w->pair_cap, w->npairs);
+  }
   hipLaunchKernelGGL(k_bp_reset, dim3(nb), dim3(256), 0, s, *P, w->hash, w->count, w->current);
   LBDEM_CHECK_LAUNCH(who);
   return LBDEM_OK;
@@ -1138,8 +1259,10 @@ int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbd
   const unsigned nb = (unsigned)((n + 255) / 256);
   const unsigned pb = 2048;  // grid-stride over the candidate list
   for (int sub = 0; sub < nsub; ++sub) {
-    rc = run_broadphase(P, S->pos, w, S->force, S->moment, s, "lbdem_dem_run");
+    rc = run_broadphase(P, S->pos, w, S->force, S->moment, s, "lbdem_dem_run", S->radius);
     if (rc) return rc;
+    if (P->probe27) hipLaunchKernelGGL(k_dem_release<27>, dim3(nb), dim3(256), 0, s, *P, *S, w->hash);
+    else hipLaunchKernelGGL(k_dem_release<8>, dim3(nb), dim3(256), 0, s, *P, *S, w->hash);
     hipLaunchKernelGGL(k_dem_resolve_existing, dim3(pb), dim3(256), 0, s, *P, *S, w->pairs, w->npairs, w->pair_cap,
                        w->need_new);
     hipLaunchKernelGGL(k_dem_resolve_new, dim3(pb), dim3(256), 0, s, *P, *S, w->pairs, w->npairs, w->pair_cap,

@@ -20,7 +20,7 @@ struct IbmArgs {
   double dx, dt, rho, dmin0, dmin1, dmin2;
   double nu_lu;
   float *volfrac, *velsum, *wsum, *velsolid, *fvel, *frho, *hforce;
-  double *force_out;
+  double *force_out, *moment_out;
   int32_t *status;
   const uint32_t *key;  // hybrid coupling: coarse-grain keys of the PSC mapping
 };
@@ -370,7 +370,10 @@ __global__ __launch_bounds__(256) void k_hyb_drag(const IbmArgs a) {
     for (int m = 0; m < 3; ++m) F[m] = -3.0 * pi * dp * mu * (1.0 - ep) * Cd * us[m];
   }
 #pragma unroll
-  for (int m = 0; m < 3; ++m) a.force_out[3 * p + m] = F[m];
+  for (int m = 0; m < 3; ++m) {
+    a.force_out[3 * p + m] = F[m];
+    a.moment_out[3 * p + m] = 0.0;  // force_fluid / moment_fluid.fill(0.0), :609-610
+  }
 }
 
 // compute_fine_weights (hybridlattice.py:378-398) for inspection
@@ -657,21 +660,27 @@ int lbdem_hybrid_map_fine(const lbdem_lattice *lat, const uint8_t *flags, const 
   return LBDEM_OK;
 }
 
-int lbdem_hybrid_fine_drag(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
-                           const lbdem_units *u, const uint32_t *key, const float *fine_volfrac, const float *vel,
-                           void *stream) {
+int lbdem_hybrid_forces(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g, const lbdem_units *u,
+                        const int32_t *coarse, int32_t ncoarse, const uint32_t *key, const float *cell_force,
+                        uint32_t *stale_key, const int32_t *stale_bbox, const float *fine_volfrac, const float *vel,
+                        void *stream) {
   IbmArgs a;
-  int rc = fill_ibm(lat, flags, g, u, (1.0 / lat->omega0 - 0.5) / 3.0, a, "lbdem_hybrid_fine_drag");
+  int rc = fill_ibm(lat, flags, g, u, (1.0 / lat->omega0 - 0.5) / 3.0, a, "lbdem_hybrid_forces");
   if (rc) return rc;
-  LBDEM_REQUIRE(key && fine_volfrac && vel, "lbdem_hybrid_fine_drag: null argument");
-  LBDEM_REQUIRE(g->n == 0 || (g->position && g->velocity && g->radius && g->force_fluid),
-                "lbdem_hybrid_fine_drag: grain arrays");
+  LBDEM_REQUIRE(coarse || ncoarse == 0, "lbdem_hybrid_forces: coarse grain list is required");
+  LBDEM_REQUIRE(key && cell_force && fine_volfrac && vel, "lbdem_hybrid_forces: null argument");
+  LBDEM_REQUIRE(g->n == 0 || (g->position && g->velocity && g->radius && g->force_fluid && g->moment_fluid),
+                "lbdem_hybrid_forces: grain arrays");
+  rc = launch_psc_forces(lat, g, u, coarse, ncoarse, key, cell_force, stale_key, stale_bbox, stream,
+                         "lbdem_hybrid_forces");
+  if (rc) return rc;
   a.key = key;
   a.volfrac = const_cast<float *>(fine_volfrac);
   a.fvel = const_cast<float *>(vel);
   a.force_out = g->force_fluid;
+  a.moment_out = g->moment_fluid;
   if (g->n > 0) hipLaunchKernelGGL(k_hyb_drag, dim3(nblk(g->n)), dim3(256), 0, as_stream(stream), a);
-  LBDEM_CHECK_LAUNCH("lbdem_hybrid_fine_drag");
+  LBDEM_CHECK_LAUNCH("lbdem_hybrid_forces");
   return LBDEM_OK;
 }
 

@@ -31,7 +31,12 @@ struct PscArgs {
   uint32_t *stale_key;     // keys of an earlier mapping, zeroed by the force kernel (may be null)
   const int *stale_bbox;   // that mapping's boxes
   int coarse_only;         // hybrid coupling: grains with 2R <= dx are not mapped (empty box)
+  const int32_t *glist;    // grain of each workgroup column (null: blockIdx.x itself)
 };
+
+__device__ __forceinline__ int grain_of_block(const PscArgs &a) {
+  return a.glist ? (int)a.glist[blockIdx.x] : (int)blockIdx.x;
+}
 
 // bounding box of lattice2grains (psclattice3d.py:326-337): [int(xc - r), int(xc + r + 2))
 // with r = R/dx, clamped; it contains every cell the mapping can touch.
@@ -48,9 +53,10 @@ __device__ __forceinline__ void grain_box(const PscArgs &a, int g, double &xc, d
   b[5] = min((int)(zc + r + 2), a.nz);
 }
 
-// Each grain's box is shared by kSplit workgroups (blockIdx.y), so that a few hundred grains
-// still fill the chip.
+// Each grain's box is shared by gridDim.y workgroups (blockIdx.y): kSplit, or more when there
+// are few grains (a handful of large grains still fills the chip).
 constexpr int kSplit = 8;
+inline int split_for(int ngrain) { return ngrain >= 256 ? kSplit : std::min(64, std::max(kSplit, 2048 / ngrain)); }
 constexpr int kMapChunk = 1024;  // cells classified per workgroup between two counting passes
 
 // min { x >= 0 : sqrt(x) >= r } for the correctly rounded sqrt: sqrt(x) < r <=> x < bound,
@@ -65,12 +71,12 @@ __device__ __forceinline__ double sqrt_lt_bound(double r) {
 
 // zero the keys written by the previous mapping (its boxes are in bbox)
 __global__ __launch_bounds__(256) void k_psc_clear(const PscArgs a) {
-  const int g = blockIdx.x;
+  const int g = grain_of_block(a);
   const int *b = a.bbox + 6 * g;
   const int ni = b[1] - b[0], nj = b[3] - b[2], nk = b[5] - b[4];
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
   const int tot = ni * nj * nk;
-  for (int t = blockIdx.y * 256 + threadIdx.x; t < tot; t += 256 * kSplit) {
+  for (int t = blockIdx.y * 256 + threadIdx.x; t < tot; t += 256 * gridDim.y) {
     const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
     a.key[(i * a.ny + j) * a.nz + k] = 0u;
   }
@@ -78,7 +84,7 @@ __global__ __launch_bounds__(256) void k_psc_clear(const PscArgs a) {
 
 // grains2lattice (psclattice3d.py:84-182), grain-centric
 __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
-  const int g = blockIdx.x;
+  const int g = grain_of_block(a);
   double xc, yc, zc;
   int b[6];
   grain_box(a, g, xc, yc, zc, b);
@@ -113,11 +119,12 @@ __global__ __launch_bounds__(256) void k_psc_map(const PscArgs a) {
   // about three lanes in four would idle through each count).
   __shared__ int queue[kMapChunk];
   __shared__ int qn;
-  for (int m0 = 0; (m0 * kSplit + (int)blockIdx.y) * 256 < tot; m0 += kMapChunk / 256) {
+  const int split = gridDim.y;
+  for (int m0 = 0; (m0 * split + (int)blockIdx.y) * 256 < tot; m0 += kMapChunk / 256) {
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
     for (int mm = 0; mm < kMapChunk / 256; ++mm) {
-      const int t = ((m0 + mm) * kSplit + (int)blockIdx.y) * 256 + (int)threadIdx.x;
+      const int t = ((m0 + mm) * split + (int)blockIdx.y) * 256 + (int)threadIdx.x;
       if (t >= tot) break;
       const int k = b[4] + t % nk, j = b[2] + (t / nk) % nj, i = b[0] + t / (nk * nj);
       const int c = (i * a.ny + j) * a.nz + k;
@@ -157,7 +164,7 @@ constexpr int kForceThreads = 1024;
 // reduction order (per-thread sums, wavefront shuffles, then the 16 wave sums pairwise in LDS),
 // so the result is deterministic.
 __global__ __launch_bounds__(kForceThreads) void k_psc_forces(const PscArgs a) {
-  const int g = blockIdx.x;
+  const int g = grain_of_block(a);
   double xc, yc, zc;
   int b[6];
   grain_box(a, g, xc, yc, zc, b);
@@ -281,6 +288,31 @@ static int fill_psc(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem
   return LBDEM_OK;
 }
 
+int launch_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u, const int32_t *glist,
+                      int32_t nlist, const uint32_t *key, const float *cell_force, uint32_t *stale_key,
+                      const int32_t *stale_bbox, void *stream, const char *who) {
+  PscArgs a;
+  int rc = fill_psc(lat, g, u, a, who);
+  if (rc) return rc;
+  LBDEM_REQUIRE(key && cell_force, "%s: key and cell_force are required", who);
+  LBDEM_REQUIRE(g->n == 0 || (g->position && g->radius && g->force_fluid && g->moment_fluid),
+                "%s: grain position, radius, force_fluid and moment_fluid are required", who);
+  LBDEM_REQUIRE(!glist || (nlist >= 0 && nlist <= g->n), "%s: grain list of %d entries for %d grains", who, nlist,
+                g->n);
+  const int ncol = glist ? nlist : g->n;
+  if (ncol == 0) return LBDEM_OK;
+  LBDEM_REQUIRE(!stale_key || (stale_bbox && stale_key != key), "%s: stale_key needs stale_bbox and must differ "
+                "from key", who);
+  a.key = const_cast<uint32_t *>(key);
+  a.cell_force = cell_force;
+  a.stale_key = stale_key;
+  a.stale_bbox = stale_bbox;
+  a.glist = glist;
+  hipLaunchKernelGGL(k_psc_forces, dim3(ncol), dim3(kForceThreads), 0, as_stream(stream), a);
+  LBDEM_CHECK_LAUNCH(who);
+  return LBDEM_OK;
+}
+
 }  // namespace lbdem
 
 using namespace lbdem;
@@ -289,57 +321,48 @@ extern "C" {
 
 static int psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g, const lbdem_units *u,
                    uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous, void *stream, int coarse_only,
-                   const char *who) {
+                   const int32_t *glist, int32_t nlist, const char *who) {
   PscArgs a;
   int rc = fill_psc(lat, g, u, a, who);
   if (rc) return rc;
   LBDEM_REQUIRE(flags && key && snap && bbox, "%s: flags, key, snap and bbox are required", who);
   LBDEM_REQUIRE(g->n == 0 || (g->position && g->velocity && g->omega && g->radius),
                 "%s: grain position, velocity, omega and radius are required", who);
+  LBDEM_REQUIRE(!glist || (nlist >= 0 && nlist <= g->n), "%s: grain list of %d entries for %d grains", who, nlist,
+                g->n);
   a.coarse_only = coarse_only;
+  a.glist = glist;
+  const int ncol = glist ? nlist : g->n;
   a.flags = flags;
   a.key = key;
   a.snap = snap;
   a.bbox = bbox;
-  if (g->n == 0) return LBDEM_OK;
+  if (ncol == 0) return LBDEM_OK;
   if (clear_previous) {
-    hipLaunchKernelGGL(k_psc_clear, dim3(g->n, kSplit), dim3(256), 0, as_stream(stream), a);
+    hipLaunchKernelGGL(k_psc_clear, dim3(ncol, split_for(ncol)), dim3(256), 0, as_stream(stream), a);
     LBDEM_CHECK_LAUNCH(who);
   }
-  hipLaunchKernelGGL(k_psc_map, dim3(g->n, kSplit), dim3(256), 0, as_stream(stream), a);
+  hipLaunchKernelGGL(k_psc_map, dim3(ncol, split_for(ncol)), dim3(256), 0, as_stream(stream), a);
   LBDEM_CHECK_LAUNCH(who);
   return LBDEM_OK;
 }
 
 int lbdem_psc_map(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g, const lbdem_units *u,
                   uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous, void *stream) {
-  return psc_map(lat, flags, g, u, key, snap, bbox, clear_previous, stream, 0, "lbdem_psc_map");
+  return psc_map(lat, flags, g, u, key, snap, bbox, clear_previous, stream, 0, nullptr, 0, "lbdem_psc_map");
 }
 
 int lbdem_hybrid_map_coarse(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains *g,
-                            const lbdem_units *u, uint32_t *key, float *snap, int32_t *bbox, int32_t clear_previous,
-                            void *stream) {
-  return psc_map(lat, flags, g, u, key, snap, bbox, clear_previous, stream, 1, "lbdem_hybrid_map_coarse");
+                            const lbdem_units *u, const int32_t *coarse, int32_t ncoarse, uint32_t *key, float *snap,
+                            int32_t *bbox, int32_t clear_previous, void *stream) {
+  return psc_map(lat, flags, g, u, key, snap, bbox, clear_previous, stream, 1, coarse, ncoarse,
+                 "lbdem_hybrid_map_coarse");
 }
 
 int lbdem_psc_forces(const lbdem_lattice *lat, const lbdem_grains *g, const lbdem_units *u, const uint32_t *key,
                      const float *cell_force, uint32_t *stale_key, const int32_t *stale_bbox, void *stream) {
-  PscArgs a;
-  int rc = fill_psc(lat, g, u, a, "lbdem_psc_forces");
-  if (rc) return rc;
-  LBDEM_REQUIRE(key && cell_force, "lbdem_psc_forces: key and cell_force are required");
-  LBDEM_REQUIRE(g->n == 0 || (g->position && g->radius && g->force_fluid && g->moment_fluid),
-                "lbdem_psc_forces: grain position, radius, force_fluid and moment_fluid are required");
-  if (g->n == 0) return LBDEM_OK;
-  LBDEM_REQUIRE(!stale_key || (stale_bbox && stale_key != key), "lbdem_psc_forces: stale_key needs stale_bbox "
-                "and must differ from key");
-  a.key = const_cast<uint32_t *>(key);
-  a.cell_force = cell_force;
-  a.stale_key = stale_key;
-  a.stale_bbox = stale_bbox;
-  hipLaunchKernelGGL(k_psc_forces, dim3(g->n), dim3(kForceThreads), 0, as_stream(stream), a);
-  LBDEM_CHECK_LAUNCH("lbdem_psc_forces");
-  return LBDEM_OK;
+  return launch_psc_forces(lat, g, u, nullptr, 0, key, cell_force, stale_key, stale_bbox, stream,
+                           "lbdem_psc_forces");
 }
 
 int lbdem_psc_decode(const lbdem_lattice *lat, const uint32_t *key, const float *snap, int32_t *id,

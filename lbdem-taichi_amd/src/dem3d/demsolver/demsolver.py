@@ -44,6 +44,11 @@ class _Record:
         self._owner._fields[name][self._i] = value
 
 
+# particle members the DEM kernels read but never write
+_DEM_READ_ONLY = frozenset(("ID", "groupID", "materialType", "mass", "radius", "inertia", "freeze", "fixvel",
+                            "force_fluid", "moment_fluid"))
+
+
 class StructField:
     """Taichi StructField look-alike: member fields as attributes, records by index."""
 
@@ -64,8 +69,11 @@ class StructField:
         for f in self._fields.values():
             f.sync()
 
-    def touch(self):
-        for f in self._fields.values():
+    def touch(self, skip=()):
+        """The device copies were written by a kernel (fields named in skip were only read)."""
+        for name, f in self._fields.items():
+            if name in skip:
+                continue
             f._host = None
             f.version += 1
 
@@ -357,7 +365,7 @@ class DEMSolver:
             self._verlet[:1].zero_()
         _hip.check(_hip.lib().lbdem_dem_run(cached[3], cached[4], cached[5], int(nsub), int(self.path),
                                             _hip.stream()), "lbdem_dem_run")
-        self.gf.touch()
+        self.gf.touch(skip=_DEM_READ_ONLY)
         self._radius_version = self.gf.radius.version
 
     def run_simulation(self):

@@ -11,8 +11,8 @@ order of the reference's driver (example/hybridlbdem/hybridexperiment/experiment
   collide()        the fused collide-stream with f + beta Os1 + B Os2 + (1 - B - beta) Of, which
                    also writes the coarse momentum exchange per cell, then the wet-node pass;
   stream()         nothing left;
-  lattice2grains() coarse force and torque (lbdem_psc_forces, FP64), then the fine grains'
-                   Tenneti drag from the post-stream fluid velocity (lbdem_hybrid_fine_drag).
+  lattice2grains() coarse force and torque (FP64 reduction per grain) and the fine grains'
+                   Tenneti drag from the post-stream fluid velocity (lbdem_hybrid_forces).
 """
 import ctypes
 
@@ -26,10 +26,14 @@ from ..lbm3d.lbmutils import CellType
 
 _vp = ctypes.c_void_p
 _P = ctypes.POINTER
+_i32 = ctypes.c_int32
 _hip.register({
+    "lbdem_hybrid_map_coarse": (ctypes.c_int, [_P(_hip.Lattice), _vp, _P(Grains), _P(Units), _vp, _i32, _vp, _vp,
+                                               _vp, _i32, _vp]),
     "lbdem_hybrid_map_fine": (ctypes.c_int, [_P(_hip.Lattice), _vp, _P(Grains), _P(Units), _vp, _vp, _vp, _vp,
                                              _vp, _vp, _vp]),
-    "lbdem_hybrid_fine_drag": (ctypes.c_int, [_P(_hip.Lattice), _vp, _P(Grains), _P(Units), _vp, _vp, _vp, _vp]),
+    "lbdem_hybrid_forces": (ctypes.c_int, [_P(_hip.Lattice), _vp, _P(Grains), _P(Units), _vp, _i32, _vp, _vp, _vp,
+                                           _vp, _vp, _vp, _vp]),
     "lbdem_hybrid_fine_weight": (ctypes.c_int, [_P(_hip.Lattice), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 })
 
@@ -43,7 +47,6 @@ class HybridLattice3D(PSCLattice3D):
     relaxation frequency field."""
     _skip_types = CellType.OBSTACLE | CellType.VEL_LADD | CellType.FREE_SLIP | CellType.VEL_INLET_LADD
     _coupling = _hip.COUPLE_HYBRID
-    _map_entry = "lbdem_hybrid_map_coarse"
 
     def __init__(self, Nx: int, Ny: int, Nz: int, omega: float, dx: float, dt: float, rho: float, dem_solver,
                  storage=None, smagorinsky=None):
@@ -67,6 +70,8 @@ class HybridLattice3D(PSCLattice3D):
         self._status = torch.zeros(2, dtype=torch.int32, device=dev)
         self._fine_weight_t = -1
         self._coarse_weight_t = -1
+        self._coarse_list = None
+        self._coarse_list_version = None
 
     def _collision(self):
         return BasicLattice3D._collision(self)
@@ -81,10 +86,39 @@ class HybridLattice3D(PSCLattice3D):
         """f <- feq at the collided cells (hybridlattice.py:158-171); no mapping."""
         BasicLattice3D.initialize(self)
 
+    def _coarse(self):
+        """Device list of the coarse grains (2R > dx), rebuilt when the radii change."""
+        r = self.dem.gf.radius
+        if self._coarse_list is None or self._coarse_list_version != r.version:
+            rad = r.sync()
+            lst = torch.nonzero(2.0 * rad > self.unit.dx).reshape(-1).to(torch.int32)
+            if self._coarse_list is not None and not torch.equal(lst, self._coarse_list):
+                # the keys are cleared through the listed grains' boxes: start both buffers clean
+                for k, b in zip(self._keys, self._bboxes):
+                    k.zero_()
+                    b.zero_()
+                self._dirty = [False, False]
+            self._coarse_list = lst
+            self._coarse_list_version = r.version
+        return self._coarse_list
+
     def map_coarse_grains(self):
-        """Coarse grains to packed keys (hybridlattice.py:173-234)."""
-        self.grains2lattice()
+        """Coarse grains to packed keys (hybridlattice.py:173-234), one workgroup column per
+        coarse grain; keys in the spare buffer of the two, as PSCLattice3D.grains2lattice."""
+        self._prepare()
+        g, u = self._structs()
+        lst = self._coarse()
+        b = 1 - self._cur
+        _hip.check(_hip.lib().lbdem_hybrid_map_coarse(
+            self._lat_c, _hip.ptr(self._flags), g, u, _hip.ptr(lst), int(lst.numel()), _hip.ptr(self._keys[b]),
+            _hip.ptr(self._snap), _hip.ptr(self._bboxes[b]), 1 if self._dirty[b] else 0, _hip.stream()),
+            "lbdem_hybrid_map_coarse")
+        self._cur = b
+        self._dirty[b] = True
+        self._map_count += 1
         self._coarse_weight_t = -1
+
+    grains2lattice = map_coarse_grains
 
     def map_fine_grains(self):
         """Fine-grain fractions and solid velocity (hybridlattice.py:260-376); call after
@@ -136,15 +170,21 @@ class HybridLattice3D(PSCLattice3D):
         self.hydroforce.touch()
 
     def lattice2grains(self):
-        """Coarse force and torque, then the fine grains' Tenneti drag (hybridlattice.py:595-740),
-        into force_fluid / moment_fluid (physical units, FP64)."""
-        super().lattice2grains()
+        """Coarse force and torque, and the fine grains' Tenneti drag (hybridlattice.py:595-740),
+        into force_fluid / moment_fluid (physical units, FP64).  The fine drag reads the
+        post-stream velocity field, refreshed here."""
         self._refresh_macros()
         g, u = self._structs()
-        _hip.check(_hip.lib().lbdem_hybrid_fine_drag(
-            self._lat_c, _hip.ptr(self._flags), g, u, _hip.ptr(self._key), _hip.ptr(self.fine_volfrac.dev),
-            _hip.ptr(self.vel.dev), _hip.stream()), "lbdem_hybrid_fine_drag")
+        lst = self._coarse()
+        other = 1 - self._cur
+        stale = (_hip.ptr(self._keys[other]), _hip.ptr(self._bboxes[other])) if self._dirty[other] else (None, None)
+        _hip.check(_hip.lib().lbdem_hybrid_forces(
+            self._lat_c, _hip.ptr(self._flags), g, u, _hip.ptr(lst), int(lst.numel()), _hip.ptr(self._key),
+            _hip.ptr(self._cell_force), stale[0], stale[1], _hip.ptr(self.fine_volfrac.dev), _hip.ptr(self.vel.dev),
+            _hip.stream()), "lbdem_hybrid_forces")
+        self._dirty[other] = False
         self.dem.gf.force_fluid.touch()
+        self.dem.gf.moment_fluid.touch()
 
     # ---------------------------------------------------------------------------------
     def initialize_complete(self):

@@ -36,8 +36,7 @@ _P = ctypes.POINTER
 _hip.register({
     "lbdem_psc_map": (ctypes.c_int, [_P(_hip.Lattice), _vp, _P(Grains), _P(Units), _vp, _vp, _vp, ctypes.c_int32,
                                      _vp]),
-    "lbdem_hybrid_map_coarse": (ctypes.c_int, [_P(_hip.Lattice), _vp, _P(Grains), _P(Units), _vp, _vp, _vp,
-                                               ctypes.c_int32, _vp]),
+
     "lbdem_psc_forces": (ctypes.c_int, [_P(_hip.Lattice), _P(Grains), _P(Units), _vp, _vp, _vp, _vp, _vp]),
     "lbdem_psc_decode": (ctypes.c_int, [_P(_hip.Lattice), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 })
@@ -64,8 +63,6 @@ class PSCLattice3D(BasicLattice3D):
     """Partially saturated cell (Noble-Torczynski) coupling; BGK with constant omega."""
     _skip_types = CellType.OBSTACLE | CellType.VEL_LADD | CellType.FREE_SLIP | CellType.VEL_INLET_LADD
     _coupling = _hip.COUPLE_PSC
-
-    _map_entry = "lbdem_psc_map"
 
     def __init__(self, Nx: int, Ny: int, Nz: int, omega: float, dx: float, dt: float, rho: float, demslover,
                  storage=None, smagorinsky=None):
@@ -123,9 +120,9 @@ class PSCLattice3D(BasicLattice3D):
         self._prepare()
         g, u = self._structs()
         b = 1 - self._cur
-        _hip.check(getattr(_hip.lib(), self._map_entry)(
-            self._lat_c, _hip.ptr(self._flags), g, u, _hip.ptr(self._keys[b]), _hip.ptr(self._snap),
-            _hip.ptr(self._bboxes[b]), 1 if self._dirty[b] else 0, _hip.stream()), self._map_entry)
+        _hip.check(_hip.lib().lbdem_psc_map(self._lat_c, _hip.ptr(self._flags), g, u,
+                                            _hip.ptr(self._keys[b]), _hip.ptr(self._snap), _hip.ptr(self._bboxes[b]),
+                                            1 if self._dirty[b] else 0, _hip.stream()), "lbdem_psc_map")
         self._cur = b
         self._dirty[b] = True
         self._map_count += 1
