@@ -183,6 +183,22 @@ int lbdem_lbm_read_f(const lbdem_lattice *lat, const lbdem_step_args *a, void *o
 int lbdem_lbm_write_f(const lbdem_lattice *lat, const lbdem_step_args *a, const void *in,
                       int32_t in_f64, void *stream);
 
+/* Stress from the non-equilibrium populations, BasicLattice3D.compute_stress
+ * (src/lbm3d/lbm_solver3d.py:347-405): sigma_ab = -(1 - omega/2) sum_q c_a c_b (f_q - feq_q),
+ * f the populations after a->t steps, feq and omega those of the last collision: exact with
+ * lagged Smagorinsky (kept in a->lag); otherwise the equilibrium of the current populations
+ * with omega0 (BGK), the fixed-point omega, or the VANS omega of a->volfrac.  stress: seven
+ * planes of ncell floats, xx, yy, zz, xy, xz, yz and the shear magnitude
+ * sqrt((xy^2 + xz^2 + yz^2) / 2); cells the collision skips are not written. */
+int lbdem_lbm_stress(const lbdem_lattice *lat, const lbdem_step_args *a, float *stress, void *stream);
+
+/* Mean of the shear components (xy, xz, yz) of `stress` over the FLUID cells of the plane
+ * {x_axis = position}, compute_plane_average_stress (src/lbm3d/lbm_solver3d.py:407-456);
+ * out: 3 doubles on the device, zeros for a plane without fluid cells.  One workgroup, fixed
+ * summation order. */
+int lbdem_lbm_plane_stress(const lbdem_lattice *lat, const uint8_t *flags, const float *stress, int32_t axis,
+                           int32_t position, double *out, void *stream);
+
 /* FP64 global sums over active cells of the post-stream state after t steps:
  * out[0] = sum rho, out[1..3] = sum rho*u, out[4] = sum rho|u|^2/2.  Deterministic
  * (fixed-order two-pass).  `work` must hold lbdem_reduce_work_doubles(lat) doubles.

@@ -128,6 +128,7 @@ struct KParams {
   double omega0, tau0, cs2_smag, rho0, nu_lu;
   double fx, fy, fz;
   int has_force;
+  int collision, coupling;  // LBDEM_COLL_* / LBDEM_COUPLE_* of the lattice
   int t_odd;  // layout parity of the state being read (t % 2)
   // outputs for macros / readout kernels
   float *rho_out;

@@ -58,6 +58,8 @@ static int fill_params(const lbdem_lattice *lat, const lbdem_step_args *a, KPara
   p.hforce_prev = a->hforce_prev;
   p.omega0 = lat->omega0;
   p.tau0 = 1.0 / lat->omega0;
+  p.collision = lat->collision;
+  p.coupling = lat->coupling;
   p.cs2_smag = lat->cs_smag * lat->cs_smag;
   p.rho0 = lat->rho0;
   p.nu_lu = a->nu_lu;
@@ -85,6 +87,43 @@ static int launch(KernelFn k, int ncell, const KParams &p, void *stream, const c
 }
 
 // deterministic fixed-order final sum of per-workgroup partials (one workgroup)
+// compute_plane_average_stress (src/lbm3d/lbm_solver3d.py:407-456): mean of the shear
+// components over the FLUID cells of one plane; one workgroup, fixed summation order
+__global__ __launch_bounds__(1024) void k_plane_stress(int nx, int ny, int nz, const uint8_t *flags,
+                                                       const float *stress, int axis, int pos, double *out) {
+  const size_t N = (size_t)nx * ny * nz;
+  const int n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int e = threadIdx.x; e < n1 * n2; e += 1024) {
+    const int a = e / n2, b = e - a * n2;
+    const int i = axis == 0 ? pos : a, j = axis == 0 ? a : (axis == 1 ? pos : b), k = axis == 2 ? pos : b;
+    const size_t c = ((size_t)i * ny + j) * nz + k;
+    if (!(flags[c] & LBDEM_FLAG_FLUID)) continue;
+    s[0] += (double)stress[3 * N + c];
+    s[1] += (double)stress[4 * N + c];
+    s[2] += (double)stress[5 * N + c];
+    s[3] += 1.0;
+  }
+  __shared__ double sh[4][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double v = wave_sum(s[m]);
+    if (lane == 0) sh[m][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double v = 0.0;
+    for (int w = 0; w < 16; ++w) v += sh[threadIdx.x][w];
+    sh[threadIdx.x][0] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const double cnt = sh[3][0];
+    out[threadIdx.x] = cnt > 0.0 ? sh[threadIdx.x][0] / cnt : 0.0;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_reduce_final(const double *work, int nblocks, double *out) {
   __shared__ double sh[256][5];
   double s[5] = {0, 0, 0, 0, 0};
@@ -281,6 +320,32 @@ int lbdem_lbm_boundaries(const lbdem_lattice *lat, const lbdem_step_args *a, con
     set_error("lbdem_lbm_boundaries: launch failed: %s", hipGetErrorString(e));
     return LBDEM_E_HIP;
   }
+  return LBDEM_OK;
+}
+
+int lbdem_lbm_stress(const lbdem_lattice *lat, const lbdem_step_args *a, float *stress, void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_stress");
+  if (rc) return rc;
+  LBDEM_REQUIRE(stress, "lbdem_lbm_stress: stress is required");
+  LBDEM_REQUIRE(lat->collision != LBDEM_COLL_SMAG_LAGGED || a->lag, "lbdem_lbm_stress: lagged Smagorinsky needs lag");
+  LBDEM_REQUIRE(lat->coupling != LBDEM_COUPLE_VANS || a->volfrac, "lbdem_lbm_stress: VANS needs volfrac");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_stress: bad storage %d", lat->storage);
+  p.out = stress;
+  return launch(t->stress, p.ncell, p, stream, "lbdem_lbm_stress");
+}
+
+int lbdem_lbm_plane_stress(const lbdem_lattice *lat, const uint8_t *flags, const float *stress, int32_t axis,
+                           int32_t position, double *out, void *stream) {
+  LBDEM_REQUIRE(lat && flags && stress && out, "lbdem_lbm_plane_stress: null argument");
+  LBDEM_REQUIRE(axis >= 0 && axis <= 2, "lbdem_lbm_plane_stress: axis %d is not 0, 1 or 2", axis);
+  const int ext = axis == 0 ? lat->nx : axis == 1 ? lat->ny : lat->nz;
+  LBDEM_REQUIRE(position >= 0 && position < ext, "lbdem_lbm_plane_stress: position %d outside [0, %d)", position,
+                ext);
+  hipLaunchKernelGGL(k_plane_stress, dim3(1), dim3(1024), 0, as_stream(stream), lat->nx, lat->ny, lat->nz, flags,
+                     stress, axis, position, out);
+  LBDEM_CHECK_LAUNCH("lbdem_lbm_plane_stress");
   return LBDEM_OK;
 }
 

@@ -789,6 +789,77 @@ __global__ __launch_bounds__(256) void k_boundaries(const KParams p, const int32
   scatter_state<S>(p, F, c, fl, odd, f);
 }
 
+// Stress from the non-equilibrium populations, BasicLattice3D.compute_stress
+// (src/lbm3d/lbm_solver3d.py:347-405): sigma_ab = -(1 - omega/2) sum_q c_a c_b (f_q - feq_q), f
+// the populations after t steps, feq and omega those of the last collision.  With lagged
+// Smagorinsky the lag state holds them (exact); otherwise the equilibrium of the current
+// populations, omega0 (BGK) or the fixed-point omega of the current populations.  Out: seven
+// planes of ncell floats (xx, yy, zz, xy, xz, yz, sqrt((xy^2 + xz^2 + yz^2) / 2)); cells the
+// collision skips are left untouched.
+template <typename S>
+__global__ __launch_bounds__(256) void k_stress(const KParams p) {
+  using T = typename Store<S>::T;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= p.ncell) return;
+  const uint8_t fl = p.flags[c];
+  if (!(fl & LBDEM_FLAG_ACTIVE)) return;
+  T f[Q];
+  gather<S>(p, reinterpret_cast<const S *>(p.f), c, fl, p.t_odd != 0, f);
+  T dr = T(0), jx = T(0), jy = T(0), jz = T(0);
+  T pxx = T(0), pyy = T(0), pzz = T(0), pxy = T(0), pxz = T(0), pyz = T(0);
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = qc;
+    dr += f[q];
+    acc<kCX[q]>(jx, f[q]);
+    acc<kCY[q]>(jy, f[q]);
+    acc<kCZ[q]>(jz, f[q]);
+    if constexpr (kCX[q] != 0) pxx += f[q];
+    if constexpr (kCY[q] != 0) pyy += f[q];
+    if constexpr (kCZ[q] != 0) pzz += f[q];
+    if constexpr (kCX[q] * kCY[q] == 1) pxy += f[q];
+    if constexpr (kCX[q] * kCY[q] == -1) pxy -= f[q];
+    if constexpr (kCX[q] * kCZ[q] == 1) pxz += f[q];
+    if constexpr (kCX[q] * kCZ[q] == -1) pxz -= f[q];
+    if constexpr (kCY[q] * kCZ[q] == 1) pyz += f[q];
+    if constexpr (kCY[q] * kCZ[q] == -1) pyz -= f[q];
+  });
+  const T rho = T(p.rho0) + dr;
+  T er = rho, ux = jx / rho, uy = jy / rho, uz = jz / rho, omega = T(p.omega0);
+  const size_t N = (size_t)p.ncell;
+  if (p.collision == LBDEM_COLL_SMAG_LAGGED && p.coupling != LBDEM_COUPLE_VANS) {
+    const T *lag = reinterpret_cast<const T *>(p.lag);
+    er = lag[c];
+    ux = lag[N + c];
+    uy = lag[2 * N + c];
+    uz = lag[3 * N + c];
+    omega = lag[4 * N + c];
+  }
+  // sum_q c_a c_b feq_q = rho/3 delta_ab + rho u_a u_b (D3Q19), in shifted variables
+  const T edr3 = (er - T(p.rho0)) * T(1.0 / 3.0);
+  const T nxx = pxx - (edr3 + er * ux * ux), nyy = pyy - (edr3 + er * uy * uy), nzz = pzz - (edr3 + er * uz * uz);
+  const T nxy = pxy - er * ux * uy, nxz = pxz - er * ux * uz, nyz = pyz - er * uy * uz;
+  if (p.coupling == LBDEM_COUPLE_VANS) {
+    // compute_omega (src/unlbdem/vanslattice.py:323-358)
+    const T svf = (T)p.volfrac[c];
+    const T g = T(1) + T(1.25) * svf * T(0.64) / (T(1) - svf);
+    omega = T(1) / (T(3) * T(p.nu_lu) * g * g + T(0.5));
+  } else if (p.collision == LBDEM_COLL_SMAG_FIXED) {
+    const T Pi = sqrt(T(2) * (nxx * nxx + nyy * nyy + nzz * nzz + T(2) * (nxy * nxy + nxz * nxz + nyz * nyz)));
+    const T tau0 = T(p.tau0);
+    omega = T(2) / (tau0 + sqrt(tau0 * tau0 + T(18) * T(p.cs2_smag) * Pi / rho));
+  }
+  const T r = -(T(1) - T(0.5) * omega);
+  float *out = reinterpret_cast<float *>(p.out);
+  const T sxy = r * nxy, sxz = r * nxz, syz = r * nyz;
+  out[c] = (float)(r * nxx);
+  out[N + c] = (float)(r * nyy);
+  out[2 * N + c] = (float)(r * nzz);
+  out[3 * N + c] = (float)sxy;
+  out[4 * N + c] = (float)sxz;
+  out[5 * N + c] = (float)syz;
+  out[6 * N + c] = (float)sqrt(T(0.5) * (sxy * sxy + sxz * sxz + syz * syz));
+}
+
 // per-workgroup FP64 partial sums: rho, rho u (3), rho |u|^2 / 2
 template <typename S>
 __global__ __launch_bounds__(256) void k_reduce_partial(const KParams p) {

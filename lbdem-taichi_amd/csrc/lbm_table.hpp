@@ -17,6 +17,7 @@ struct LbmKernels {
   KernelFn read_f;
   KernelFn write_f;
   BcKernelFn boundaries;
+  KernelFn stress;
   KernelFn reduce_partial;
 };
 
@@ -118,6 +119,7 @@ void add_hybrid_f64(LbmKernels &k);
       k.read_f = k_read_f<S>;                                                                   \
       k.write_f = k_write_f<S>;                                                                 \
       k.boundaries = k_boundaries<S>;                                                           \
+      k.stress = k_stress<S>;                                                                   \
       k.reduce_partial = k_reduce_partial<S>;                                                   \
       return k;                                                                                 \
     }();                                                                                        \

@@ -64,6 +64,8 @@ def lib():
         "lbdem_lbm_reduce": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, _vp]),
         "lbdem_lbm_prepare_flags": (ctypes.c_int, [P(Lattice), _vp, _vp, _vp]),
         "lbdem_lbm_boundaries": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, ctypes.c_int32, _vp, _vp, _vp]),
+        "lbdem_lbm_stress": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp]),
+        "lbdem_lbm_plane_stress": (ctypes.c_int, [P(Lattice), _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     }
     sig.update(_EXTRA_SIGNATURES)
     for name, (res, args) in sig.items():

@@ -148,6 +148,49 @@ class DEMSolver:
                        fmt="%d %d %.6e %.6e %.6e %.6e %.6e %.6e")
 
     # ---------------------------------------------------------------------------------
+    # restart snapshot (no counterpart in the reference, whose p4p / p4c output keeps neither
+    # the rotation state nor the contact history a run resumes from)
+    # ---------------------------------------------------------------------------------
+    _HISTORY = ("_cf_j", "_cf_active", "_cfn", "_cf_pos", "_cf_force", "_cf_shear", "_wcf_active", "_wcf_pos",
+                "_wcf_force", "_wcf_shear")
+
+    def state_dict(self):
+        """Every particle member (position, velocity, acceleration, force, quaternion, omega,
+        omega_dot, inertia, moment, freeze, fixvel, fluid loads, ...) and the particle-particle
+        and particle-wall contact history (partner, active flag, contact point, force, shear
+        displacement), as numpy arrays."""
+        self.gf.sync()
+        d = {f"gf.{k}": f.dev.detach().cpu().numpy() for k, f in self.gf._fields.items()}
+        for k in self._HISTORY:
+            d[k] = getattr(self, k).detach().cpu().numpy()
+        d["n"], d["K"] = np.array(self.n), np.array(self.K)
+        return d
+
+    def load_state_dict(self, d):
+        """Restore a state_dict() into a solver set up with the same particle count and
+        max_coordinate_number.  The single-workgroup path rebuilds its neighbour lists."""
+        if int(d["n"]) != self.n or int(d["K"]) != self.K:
+            raise ValueError(f"snapshot of n={int(d['n'])}, K={int(d['K'])} does not fit a solver of "
+                             f"n={self.n}, K={self.K}")
+        for k, f in self.gf._fields.items():
+            f.from_numpy(d[f"gf.{k}"])
+        for k in self._HISTORY:
+            getattr(self, k).copy_(torch.from_numpy(np.ascontiguousarray(d[k])))
+        for sf in (self.cf, self.wcf):
+            sf.touch()
+        self.cfn.touch()
+        self._verlet[:1].zero_()
+        self._radius_version = None
+
+    def save_state(self, file_name):
+        """Write state_dict() to an .npz file (no pickled objects)."""
+        np.savez(file_name, **self.state_dict())
+
+    def load_state(self, file_name):
+        with np.load(file_name, allow_pickle=False) as z:
+            self.load_state_dict({k: z[k] for k in z.files})
+
+    # ---------------------------------------------------------------------------------
     # set-up
     # ---------------------------------------------------------------------------------
     @staticmethod

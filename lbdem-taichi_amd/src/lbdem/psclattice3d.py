@@ -145,6 +145,14 @@ class PSCLattice3D(BasicLattice3D):
         self.hydroforce.touch()
         self.lattice2grains()
 
+    def _after_load(self):
+        # keys are remapped at the next collide; start both buffers clean
+        for k, b in zip(self._keys, self._bboxes):
+            k.zero_()
+            b.zero_()
+        self._dirty = [False, False]
+        self._map_count += 1
+
     def _decode(self):
         if self._decoded_version == self._map_count:
             return

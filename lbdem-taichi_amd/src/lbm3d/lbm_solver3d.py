@@ -292,6 +292,115 @@ class BasicLattice3D:
             self.stream()
 
     # ----------------------------------------------------------------------------------
+    # stress diagnostics (lbm_solver3d.py:347-456)
+    # ----------------------------------------------------------------------------------
+    _STRESS = ("stress_xx", "stress_yy", "stress_zz", "stress_xy", "stress_xz", "stress_yz", "shear_stress_mag")
+
+    def _stress_fields(self):
+        """Seven FP32 planes, allocated at the first use (28 B per cell)."""
+        if getattr(self, "_stress", None) is None:
+            shp = (self.Nx, self.Ny, self.Nz)
+            self._stress = torch.zeros((7, self.N), dtype=torch.float32, device=self.device)
+            for k, name in enumerate(self._STRESS):
+                self.__dict__[name] = Field(shp, torch.float32, self.device, tensor=self._stress[k].view(shp))
+        return self._stress
+
+    def __getattr__(self, name):
+        if name in BasicLattice3D._STRESS:
+            self._stress_fields()
+            return self.__dict__[name]
+        raise AttributeError(name)
+
+    def compute_stress(self):
+        """sigma = -(1 - omega/2) sum_q c c (f - feq) per collided cell, with feq and omega of the
+        last collision (exact with lagged Smagorinsky; otherwise those of the current
+        populations), into stress_xx ... stress_yz and shear_stress_mag (:347-405)."""
+        self._prepare()
+        st = self._stress_fields()
+        _hip.check(_hip.lib().lbdem_lbm_stress(self._lat_c, self._args_c(), _hip.ptr(st), _hip.stream()),
+                   "lbdem_lbm_stress")
+        for name in self._STRESS:
+            self.__dict__[name].touch()
+
+    def compute_plane_average_stress(self, axis: int, position: int):
+        """(mean xy, mean xz, mean yz) shear stress over the FLUID cells of the plane
+        {axis = position} (:407-456), from the last compute_stress."""
+        st = self._stress_fields()
+        out = torch.empty(3, dtype=torch.float64, device=self.device)
+        _hip.check(_hip.lib().lbdem_lbm_plane_stress(self._lat_c, _hip.ptr(self._flags), _hip.ptr(st), int(axis),
+                                                     int(position), _hip.ptr(out), _hip.stream()),
+                   "lbdem_lbm_plane_stress")
+        return tuple(float(v) for v in out.cpu())
+
+    # ----------------------------------------------------------------------------------
+    # restart snapshot (no counterpart in the reference)
+    # ----------------------------------------------------------------------------------
+    def _snapshot_fields(self):
+        """Fields a run resumes from: inputs and state, not those derived on request."""
+        seen, out = set(), {}
+        for name, val in vars(self).items():
+            if not isinstance(val, Field) or id(val) in seen or name in self._STRESS:
+                continue
+            seen.add(id(val))
+            if val._refresh is not None and val is not self.rho and val is not self.vel:
+                continue  # decoded / inspection fields, recomputed from the state
+            out[name] = val
+        return out
+
+    def state_dict(self):
+        """The populations (the stored array, and unshifted FP64 for a restore into another
+        storage type), the step counter, the lagged-Smagorinsky state and every input or state
+        field (CT, rho, vel, omega, coupling fields), as numpy arrays."""
+        self._prepare()
+        pops = torch.empty((19, self.N), dtype=torch.float64, device=self.device)
+        _hip.check(_hip.lib().lbdem_lbm_read_f(self._lat_c, self._args_c(), _hip.ptr(pops), 1, _hip.stream()),
+                   "lbdem_lbm_read_f")
+        d = {"f": pops.cpu().numpy(), "t": np.array(self.t), "shape": np.array([self.Nx, self.Ny, self.Nz]),
+             "storage": np.array(self.storage),
+             # the stored array itself (its layout follows the parity and the walls): exact restore
+             # into a lattice of the same storage type and cell types
+             "f_raw": self._f.view(torch.uint8).cpu().numpy()}
+        if self._lag is not None:
+            d["lag"] = self._lag.cpu().numpy()
+        for name, f in self._snapshot_fields().items():
+            d["field." + name] = f.to_numpy()
+        return d
+
+    def load_state_dict(self, d):
+        """Restore a state_dict() into a lattice of the same shape and class: the next step
+        continues exactly where the snapshot was taken."""
+        if tuple(int(v) for v in d["shape"]) != (self.Nx, self.Ny, self.Nz):
+            raise ValueError(f"snapshot of shape {tuple(d['shape'])} does not fit {(self.Nx, self.Ny, self.Nz)}")
+        fields = self._snapshot_fields()
+        self.CT.from_numpy(d["field.CT"])
+        self._populated = False  # the populations are written below, after the flags
+        self.t = int(d["t"])
+        self._prepare()
+        if "f_raw" in d and str(d["storage"]) == self.storage:
+            self._f.view(torch.uint8).copy_(torch.from_numpy(np.ascontiguousarray(d["f_raw"])))
+            self._populated = True
+        else:  # another storage type: through the unshifted FP64 populations
+            self._write_populations(torch.from_numpy(np.ascontiguousarray(d["f"])).to(self.device))
+        if self._lag is not None and "lag" in d:
+            self._lag.copy_(torch.from_numpy(np.ascontiguousarray(d["lag"])))
+        for name, f in fields.items():
+            if name != "CT" and "field." + name in d:
+                f.from_numpy(d["field." + name])
+        self._macro_t = self.t
+        self._after_load()
+
+    def _after_load(self):
+        """Subclasses reset state derived from what was restored."""
+
+    def save_state(self, file_name):
+        """Write state_dict() to an .npz file (no pickled objects)."""
+        np.savez(file_name, **self.state_dict())
+
+    def load_state(self, file_name):
+        with np.load(file_name, allow_pickle=False) as z:
+            self.load_state_dict({k: z[k] for k in z.files})
+
+    # ----------------------------------------------------------------------------------
     # diagnostics
     # ----------------------------------------------------------------------------------
     def _reduce(self):

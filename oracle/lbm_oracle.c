@@ -1000,3 +1000,50 @@ void SFX(lbo_hybrid_fine_drag)(int nx, int ny, int nz, const int32_t *CT, int32_
     for (int a = 0; a < 3; ++a) out[3 * p + a] = F[a];
   }
 }
+
+/* ------------------------------------------------------------------------------------- */
+/* BasicLattice3D.compute_stress, src/lbm3d/lbm_solver3d.py:347-405: f the post-stream
+ * populations, feq and omega those the last collision left; out: 7 planes (xx, yy, zz, xy, xz,
+ * yz, shear magnitude); skipped cells untouched. */
+void SFX(lbo_stress)(int nx, int ny, int nz, const int32_t *CT, const REAL *f, const REAL *feq, const REAL *omega,
+                     REAL *out) {
+  const long n = (long)nx * ny * nz;
+  const int32_t skip = CT_OBSTACLE | CT_VEL_LADD | CT_FREE_SLIP | CT_VEL_INLET_LADD;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < n; ++c) {
+    if (CT[c] & skip) continue;
+    const REAL rf = 1.0 - 0.5 * omega[c];
+    REAL s[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < Q; ++q) {
+      const REAL fneq = f[Q * c + q] - feq[Q * c + q];
+      s[0] += CX[q] * CX[q] * fneq;
+      s[1] += CY[q] * CY[q] * fneq;
+      s[2] += CZ[q] * CZ[q] * fneq;
+      s[3] += CX[q] * CY[q] * fneq;
+      s[4] += CX[q] * CZ[q] * fneq;
+      s[5] += CY[q] * CZ[q] * fneq;
+    }
+    for (int m = 0; m < 6; ++m) out[m * n + c] = -rf * s[m];
+    out[6 * n + c] = sqrt(0.5 * (out[3 * n + c] * out[3 * n + c] + out[4 * n + c] * out[4 * n + c] +
+                                 out[5 * n + c] * out[5 * n + c]));
+  }
+}
+
+/* compute_plane_average_stress, src/lbm3d/lbm_solver3d.py:407-456: means of the xy, xz, yz
+ * components over the cells of the plane whose CellType has the FLUID bit */
+void SFX(lbo_plane_stress)(int nx, int ny, int nz, const int32_t *CT, const REAL *st, int axis, int pos,
+                           double *out) {
+  const long n = (long)nx * ny * nz;
+  double s[3] = {0, 0, 0};
+  long cnt = 0;
+  const int n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
+  for (int a = 0; a < n1; ++a)
+    for (int b = 0; b < n2; ++b) {
+      const int i = axis == 0 ? pos : a, j = axis == 0 ? a : (axis == 1 ? pos : b), k = axis == 2 ? pos : b;
+      const long c = IDX(i, j, k);
+      if (!(CT[c] & CT_FLUID)) continue;
+      for (int m = 0; m < 3; ++m) s[m] += st[(3 + m) * n + c];
+      ++cnt;
+    }
+  for (int m = 0; m < 3; ++m) out[m] = cnt > 0 ? s[m] / (double)cnt : 0.0;
+}

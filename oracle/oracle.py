@@ -315,6 +315,22 @@ class HybridOracle:
         return hybrid_forces(self.L, self.g, self.dmin, self.dx, self.dt, self.rho_phys)
 
 
+def stress(L):
+    """BasicLattice3D.compute_stress (lbm_solver3d.py:347-405) -> (7, nx, ny, nz): xx, yy, zz, xy,
+    xz, yz, shear magnitude; zeros at skipped cells."""
+    out = np.zeros((7,) + L.CT.shape, dtype=L.real)
+    _fn("lbo_stress", L.real)(*L._g(), P(L.CT), P(L.f), P(L.feq), P(L.omega), P(out))
+    return out
+
+
+def plane_stress(L, st, axis, position):
+    """compute_plane_average_stress (lbm_solver3d.py:407-456) -> (xy, xz, yz) means."""
+    out = np.zeros(3)
+    _fn("lbo_plane_stress", L.real)(*L._g(), P(L.CT), P(np.ascontiguousarray(st, dtype=L.real)),
+                                    ctypes.c_int(axis), ctypes.c_int(position), P(out))
+    return tuple(out)
+
+
 def cavity(L, lid_u, lid_axis=2):
     """Lid-driven cavity cell types (lid_driven_cavity.py:31-47 with the lid on the +lid_axis face)."""
     CT = L.CT
