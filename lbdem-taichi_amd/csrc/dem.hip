@@ -490,23 +490,45 @@ __device__ __forceinline__ void resolve_new(const lbdem_dem_params &P, const lbd
 // does not touch changes nothing unless it holds a history slot, which k_dem_release clears;
 // so the outcome is that of resolving every candidate, while the list stays the size of the
 // contact set (a strongly polydisperse packing, whose cells are 4 r_max wide, has O(n^2)
-// candidates).  blockIdx.y splits each particle's candidates over gridDim.y lanes.
+// candidates).  blockIdx.y splits each particle's candidates over gridDim.y workgroups.
+//
+// When the probed buckets of the workgroup's particles number at most kTileBuckets (a coarse
+// grid puts whole workgroups into the same few buckets), the buckets are walked together:
+// 256 candidates at a time are staged in LDS and every lane tests them from there (lanes test
+// only buckets of their own probe set).  Otherwise each lane walks its own buckets.
+constexpr int kTileBuckets = 64;
+constexpr int kTileTab = 128;  // LDS hash set collecting the workgroup's buckets
+
+__device__ __forceinline__ void emit_pair(int i, int j, int32_t *pairs, int64_t cap, unsigned long long *npairs) {
+  const unsigned long long w = atomicAdd(npairs, 1ull);
+  if (w < (unsigned long long)cap) {
+    pairs[2 * w] = i;
+    pairs[2 * w + 1] = j;
+  }
+}
+
 template <int NP>
 __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P, const double *pos,
                                                         const double *rad, const int32_t *count,
                                                         const int32_t *offset, const int32_t *pid, int32_t *pairs,
                                                         int64_t cap, unsigned long long *npairs) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P.n) return;
+  const bool active = i < P.n;
   const int split = gridDim.y, part = blockIdx.y;
-  const double *xi = pos + 3 * i;
-  int32_t c[3];
-  cell_of(P, xi, c);
   int32_t h[NP];
-  probe_cells<NP>(P, xi, c, h);
+  double xi[3] = {0.0, 0.0, 0.0}, ri = 0.0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) h[k] = -1;
+  if (active) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xi[a] = pos[3 * i + a];
+    ri = rad[i];
+    int32_t c[3];
+    cell_of(P, xi, c);
+    probe_cells<NP>(P, xi, c, h);
+  }
   if constexpr (NP == 8) {
-    // ascending bucket order: lanes whose probe sets share buckets (a coarse grid) walk them in
-    // step, so their candidate loads coincide
+    // ascending bucket order: lanes whose probe sets share buckets walk them in step
 #pragma unroll
     for (int a = 0; a < NP - 1; ++a)
 #pragma unroll
@@ -516,7 +538,67 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
         h[b + 1] = hi;
       }
   }
-  const double ri = rad[i];
+  __shared__ int32_t s_tab[kTileTab];
+  __shared__ int32_t s_list[kTileBuckets];
+  __shared__ int s_nb, s_over;
+  __shared__ double s_x[256], s_y[256], s_z[256], s_r[256];
+  __shared__ int32_t s_j[256];
+  if (threadIdx.x < kTileTab) s_tab[threadIdx.x] = -1;
+  if (threadIdx.x == 0) s_nb = s_over = 0;
+  __syncthreads();
+  // the workgroup's distinct buckets (hashes are >= 0; -1 marks a free slot)
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    if (h[k] < 0) continue;
+    int slot = h[k] & (kTileTab - 1);
+    bool placed = false;
+    for (int t = 0; t < kTileTab && !placed; ++t) {
+      const int32_t old = atomicCAS(&s_tab[slot], -1, h[k]);
+      placed = (old == -1 || old == h[k]);
+      slot = (slot + 1) & (kTileTab - 1);
+    }
+    if (!placed) s_over = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x < kTileTab && s_tab[threadIdx.x] >= 0) {
+    const int u = atomicAdd(&s_nb, 1);
+    if (u < kTileBuckets) s_list[u] = s_tab[threadIdx.x];
+  }
+  __syncthreads();
+  const int nb = s_nb;
+  if (s_over == 0 && nb <= kTileBuckets) {
+    for (int u = 0; u < nb; ++u) {
+      const int32_t bk = s_list[u];
+      bool mine = false;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) mine |= (h[k] == bk);
+      const int32_t b = offset[bk], e = b + count[bk];
+      for (int base = b + part * 256; base < e; base += split * 256) {
+        __syncthreads();  // the previous chunk has been read
+        const int t = base + (int)threadIdx.x;
+        if (t < e) {
+          const int32_t j = pid[t];
+          s_j[threadIdx.x] = j;
+          s_x[threadIdx.x] = pos[3 * j];
+          s_y[threadIdx.x] = pos[3 * j + 1];
+          s_z[threadIdx.x] = pos[3 * j + 2];
+          s_r[threadIdx.x] = rad[j];
+        }
+        __syncthreads();
+        if (active && mine) {
+          const int m = min(256, e - base);
+          for (int q = 0; q < m; ++q) {
+            const int32_t j = s_j[q];
+            if (j <= i) continue;
+            const double xj[3] = {s_x[q], s_y[q], s_z[q]};
+            if (touching(P, xi, ri, xj, s_r[q])) emit_pair(i, j, pairs, cap, npairs);
+          }
+        }
+      }
+    }
+    return;
+  }
+  if (!active) return;
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
     if (h[k] < 0) continue;
@@ -524,11 +606,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
     for (int idx = b + part; idx < e; idx += split) {
       const int32_t j = pid[idx];
       if (j <= i || !touching(P, xi, ri, pos + 3 * j, rad[j])) continue;
-      const unsigned long long w = atomicAdd(npairs, 1ull);
-      if (w < (unsigned long long)cap) {
-        pairs[2 * w] = i;
-        pairs[2 * w + 1] = j;
-      }
+      emit_pair(i, j, pairs, cap, npairs);
     }
   }
 }

@@ -485,12 +485,15 @@ static int cmp_i64(const void *a, const void *b) {
 /* run_simulation, demsolver.py:848-862 (clear_state :422-429, contact :840-846,
  * resolve_wall :809-838, apply_body_force :437-448, update :514-636,
  * late_clear_state :431-436 with remove_inactive_contact :371-386).
- * pairs: scratch of capacity cap pairs; returns the candidate count of the step. */
+ * pairs: scratch of capacity cap pairs; returns the candidate count of the step, or minus
+ * that count, with nothing but the force / moment clearing done, when it exceeds cap (the
+ * caller grows the scratch and repeats the step). */
 int64_t lbo_dem_step(const dem_params *P, dem_state *S, int32_t *pairs, int64_t cap) {
   const int n = P->n, K = P->K;
   for (int k = 0; k < 3 * n; ++k) S->force[k] = S->moment[k] = 0.0;
   const int64_t np = lbo_broadphase(n, S->pos, P->dmin, P->cell, P->T, P->probe27, pairs, cap);
-  const int64_t use = np < cap ? np : cap;
+  if (np > cap) return -np;
+  const int64_t use = np;
   /* resolve in the bit table's (i, j) order (demsolver.py:412-417) */
   int64_t *keys = malloc(sizeof(int64_t) * (size_t)(use > 0 ? use : 1));
   for (int64_t p = 0; p < use; ++p) keys[p] = (int64_t)pairs[2 * p] * n + pairs[2 * p + 1];

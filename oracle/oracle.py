@@ -482,8 +482,12 @@ class OracleDEM:
         for k in _STATE_FIELDS:
             setattr(s, k, P(self.a[k]))
         for _ in range(n):
-            L.lbo_dem_step(ctypes.byref(self.params), ctypes.byref(s), P(self._pairs),
-                           ctypes.c_int64(len(self._pairs)))
+            while True:
+                r = L.lbo_dem_step(ctypes.byref(self.params), ctypes.byref(s), P(self._pairs),
+                                   ctypes.c_int64(len(self._pairs)))
+                if r >= 0:
+                    break
+                self._pairs = np.zeros((-r + 1024, 2), dtype=np.int32)  # more candidates than room
 
     def __getitem__(self, k):
         return self.a[k]

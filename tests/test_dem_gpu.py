@@ -89,6 +89,37 @@ def test_dem_trajectories_match_oracle(model, path):
     assert gpu.status()["slot_overflow"] == 0
 
 
+def test_dem_coarse_grid_polydisperse_matches_oracle():
+    """Two grains of R 0.13 / 0.11 among ~2,900 of R <= 0.02: the broad-phase cell (4 r_max)
+    makes a 2 x 2 x 2 grid, so each particle has ~1,000 candidates and the multi-kernel probe
+    stages whole buckets in LDS.  Trajectories and contact sets follow the oracle."""
+    rng = np.random.default_rng(3)
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    big = np.array([[0.3, 0.35, 0.4], [0.68, 0.62, 0.55]])
+    Rb = np.array([0.13, 0.11])
+    rad_s, _, pos_s, vel_s = random_packing(3000, 0.02, box, rng, v=2.0, polydisperse=0.3)
+    keep = np.all(np.linalg.norm(pos_s[:, None, :] - big[None], axis=2) > Rb[None] + rad_s[:, None] + 0.005, axis=1)
+    pos = np.vstack([big, pos_s[keep]])
+    rad = np.concatenate([Rb, rad_s[keep]])
+    mass = 2500.0 * 4 / 3 * np.pi * rad ** 3
+    vel = np.vstack([np.zeros((2, 3)), vel_s[keep]])
+    cfg = make_config(model="hertz", box=box, dt=5e-5)
+    gpu = gpu_solver(cfg, rad, mass, pos, vel, path=2)
+    orc = oracle_solver(cfg, rad, mass, pos, vel)
+    for _ in range(4):
+        gpu.run_substeps(50)
+        orc.step(50)
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+    assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
+    K = gpu.K
+    act = gpu._cf_active.cpu().numpy().astype(bool)
+    got = sorted(zip(np.repeat(np.arange(gpu.n), K)[act].tolist(), gpu._cf_j.cpu().numpy()[act].tolist()))
+    oact = orc["cf_active"].astype(bool)
+    ref = sorted(zip(np.repeat(np.arange(orc.n), K)[oact].tolist(), orc["cf_j"][oact].tolist()))
+    assert got == ref and len(ref) > 20
+    gpu.check()
+
+
 def test_dem_anisotropic_inertia_matches_oracle():
     """Full inertia tensors (principal moments 1 : 1.4 : 0.7, randomly rotated): the general
     rotation update (body frame, inverse inertia, Euler term), where spheres take the isotropic

@@ -116,3 +116,20 @@ def test_oracle_packing_settles_without_slot_overflow():
     o.step(300)
     assert o["status"][0] == 0
     assert np.isfinite(o["pos"]).all()
+
+
+def test_oracle_step_grows_its_candidate_scratch():
+    """A step whose candidates outnumber the scratch is repeated with a larger one: the result
+    is that of a scratch large enough from the start (no candidate is dropped)."""
+    rng = np.random.default_rng(4)
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    rad, mass, pos, vel = random_packing(150, 0.04, box, rng, v=2.0, polydisperse=0.3)
+    cfg = make_config(model="hertz", box=box, dt=5e-5)
+    a = oracle_solver(cfg, rad, mass, pos, vel)
+    b = oracle_solver(cfg, rad, mass, pos, vel)
+    b._pairs = np.zeros((8, 2), dtype=np.int32)
+    a.step(60)
+    b.step(60)
+    assert len(b._pairs) > 8
+    for k in ("pos", "vel", "omega", "cf_active", "cf_j"):
+        assert np.array_equal(a[k], b[k]), k
