@@ -498,6 +498,7 @@ __device__ __forceinline__ void resolve_new(const lbdem_dem_params &P, const lbd
 // only buckets of their own probe set).  Otherwise each lane walks its own buckets.
 constexpr int kTileBuckets = 64;
 constexpr int kTileTab = 128;  // LDS hash set collecting the workgroup's buckets
+constexpr int kTileProbe = 8;  // linear-probing steps per insert before the set counts as full
 
 __device__ __forceinline__ void emit_pair(int i, int j, int32_t *pairs, int64_t cap, unsigned long long *npairs) {
   const unsigned long long w = atomicAdd(npairs, 1ull);
@@ -546,18 +547,26 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
   if (threadIdx.x < kTileTab) s_tab[threadIdx.x] = -1;
   if (threadIdx.x == 0) s_nb = s_over = 0;
   __syncthreads();
-  // the workgroup's distinct buckets (hashes are >= 0; -1 marks a free slot)
+  // the workgroup's distinct buckets (hashes are >= 0; -1 marks a free slot), for the 8-cell
+  // probe only (the 27-cell probe serves fine grids, where the set would overflow anyway).  An
+  // insert gives up after kTileProbe slots and marks the set as overflowed, and a lane stops
+  // inserting once it sees the mark, so a sparse grid costs a few LDS atomics per lane.
+  if constexpr (NP == 8) {
 #pragma unroll
-  for (int k = 0; k < NP; ++k) {
-    if (h[k] < 0) continue;
-    int slot = h[k] & (kTileTab - 1);
-    bool placed = false;
-    for (int t = 0; t < kTileTab && !placed; ++t) {
-      const int32_t old = atomicCAS(&s_tab[slot], -1, h[k]);
-      placed = (old == -1 || old == h[k]);
-      slot = (slot + 1) & (kTileTab - 1);
+    for (int k = 0; k < NP; ++k) {
+      if (h[k] < 0) continue;
+      if (*(volatile int *)&s_over) break;
+      int slot = h[k] & (kTileTab - 1);
+      bool placed = false;
+      for (int t = 0; t < kTileProbe && !placed; ++t) {
+        const int32_t old = atomicCAS(&s_tab[slot], -1, h[k]);
+        placed = (old == -1 || old == h[k]);
+        slot = (slot + 1) & (kTileTab - 1);
+      }
+      if (!placed) s_over = 1;
     }
-    if (!placed) s_over = 1;
+  } else {
+    if (threadIdx.x == 0) s_over = 1;
   }
   __syncthreads();
   if (threadIdx.x < kTileTab && s_tab[threadIdx.x] >= 0) {
