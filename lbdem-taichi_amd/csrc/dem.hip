@@ -243,6 +243,14 @@ __global__ __launch_bounds__(256) void k_bp_reset(const lbdem_dem_params P, cons
 // --------------------------------------------------------------------------------------
 // contact mechanics
 // --------------------------------------------------------------------------------------
+// status counters (overlap and time-step warnings, slot overflow): the lanes of a wavefront that
+// count together take one atomic (a dense bed raises hundreds of thousands per substep, and one
+// atomic each on a single address serialises them)
+__device__ __forceinline__ void count_event(int32_t *ctr) {
+  const unsigned long long m = __ballot(1);
+  if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) atomicAdd(ctr, (int32_t)__popcll(m));
+}
+
 enum { MAT_RHO, MAT_E, MAT_NU, MAT_MU, MAT_REST, MAT_KN, MAT_KT, MAT_DPN, MAT_DPS };
 
 __device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
@@ -332,7 +340,7 @@ __device__ __forceinline__ void contact_law(const lbdem_dem_params &P, const lbd
       const double ri = S.radius[i];
       const double rho = S.mass[i] * 3 / (4 * pi * ri * ri * ri);
       const double tmin = pi * Rs * sqrt(rho / Gs) / (0.1631 * mi[MAT_NU] + 0.8766);  // hertz.py:156-162
-      if (dt > tmin) atomicAdd(S.status + 1, 1);
+      if (dt > tmin) count_event(S.status + 1);
       ms = 1.0 / (1.0 / S.mass[i] + 1.0 / S.mass[j]);
       e = 0.5 * (mi[MAT_REST] + mj[MAT_REST]);
       mu = 0.5 * (mi[MAT_MU] + mj[MAT_MU]);
@@ -359,7 +367,7 @@ __device__ __forceinline__ void contact_law(const lbdem_dem_params &P, const lbd
       kt = mi[MAT_KT] * mj[MAT_KT] / (mi[MAT_KT] + mj[MAT_KT]);
       dpn = 0.5 * (mi[MAT_DPN] + mj[MAT_DPN]);
       dps = 0.5 * (mi[MAT_DPS] + mj[MAT_DPS]);
-      if (dt > sqrt(S.mass[i] / mi[MAT_KN])) atomicAdd(S.status + 1, 1);  // linear.py:155-160
+      if (dt > sqrt(S.mass[i] / mi[MAT_KN])) count_event(S.status + 1);  // linear.py:155-160
       ms = 1.0 / (1.0 / S.mass[i] + 1.0 / S.mass[j]);
       mu = 0.5 * (mi[MAT_MU] + mj[MAT_MU]);
     }
@@ -390,7 +398,7 @@ __device__ __forceinline__ void eval_pair(const lbdem_dem_params &P, const lbdem
   rot_to_x(a, R);
   const double delta_n = -gap;
   const double Rmin = fmin(S.radius[i], S.radius[j]);
-  if (delta_n > 0.05 * Rmin) atomicAdd(S.status + 2, 1);
+  if (delta_n > 0.05 * Rmin) count_event(S.status + 2);
   const double *xi = S.pos + 3 * i;
   double cp[3], ri[3], rj[3], vci[3], vcj[3], dv[3], vc[3], F[3], sh[3];
 #pragma unroll
@@ -471,7 +479,7 @@ __device__ __forceinline__ void resolve_new(const lbdem_dem_params &P, const lbd
   const int K = P.K;
   const int slot = atomicAdd(S.cfn + i, 1);
   if (slot >= K) {  // the reference prints and writes cf[-1]; here: counted, no write
-    atomicAdd(S.status + 0, 1);
+    count_event(S.status + 0);
     return;
   }
   const int off = i * K + slot;
@@ -711,7 +719,7 @@ __device__ __forceinline__ void resolve_walls(const lbdem_dem_params &P, const l
     rot_to_x(nrm, R);
     const double delta_n = pd + S.radius[i];
     const double distance = S.radius[i] - delta_n;
-    if (delta_n > 0.05 * S.radius[i]) atomicAdd(S.status + 3, 1);
+    if (delta_n > 0.05 * S.radius[i]) count_event(S.status + 3);
     double ri[3], vci[3], mv[3], vc[3], F[3], sh[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
