@@ -240,65 +240,62 @@ __device__ __forceinline__ double coarse_eps(uint32_t key) {
   return (double)((key >> 23) & 0x7Fu) / 125.0;
 }
 
-// map_fine_grains passes 1-3 (hybridlattice.py:284-361), one thread per grain with 2R <= dx:
-// kernel sum, free-fraction correction sum, then the spread (FP32 atomics)
-__global__ __launch_bounds__(256) void k_hyb_spread(const IbmArgs a) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= a.np) return;
-  const double R = a.rad[p];
-  if (2.0 * R > a.dx) return;  // coarse grain (PSC side)
-  double xc, yc, zc;
-  lattice_pos(a, p, xc, yc, zc);
+// map_fine_grains passes 1-3 (hybridlattice.py:284-361), one wavefront per grain with
+// 2R <= dx: lane l takes cell l of the grain's box (at most 4 x 4 x 4 cells), the kernel sum and
+// the free-fraction correction sum are wave reductions, then each lane spreads into its cell
+// (FP32 atomics).  The sums are trees instead of the reference's serial loops (round-off only).
+constexpr int kGrainsPerBlock = 4;  // 256 threads
+
+struct BoxLane {
+  bool in;   // the lane's cell lies in the box
+  int c;     // its cell index
+  int i, j, k;
+};
+
+__device__ __forceinline__ BoxLane box_lane(const IbmArgs &a, double xc, double yc, double zc) {
   int i0, i1, j0, j1, k0, k1;
   eq_box(xc, a.nx, i0, i1);
   eq_box(yc, a.ny, j0, j1);
   eq_box(zc, a.nz, k0, k1);
+  const int l = threadIdx.x & 63;
+  BoxLane b;
+  b.i = i0 + (l >> 4);
+  b.j = j0 + ((l >> 2) & 3);
+  b.k = k0 + (l & 3);
+  b.in = b.i < i1 && b.j < j1 && b.k < k1;
+  b.c = b.in ? (b.i * a.ny + b.j) * a.nz + b.k : 0;
+  return b;
+}
+
+__global__ __launch_bounds__(256) void k_hyb_spread(const IbmArgs a) {
+  const int p = blockIdx.x * kGrainsPerBlock + (int)(threadIdx.x >> 6);
+  if (p >= a.np) return;  // uniform over the wavefront
+  const double R = a.rad[p];
+  if (2.0 * R > a.dx) return;  // coarse grain (PSC side)
+  double xc, yc, zc;
+  lattice_pos(a, p, xc, yc, zc);
+  const BoxLane b = box_lane(a, xc, yc, zc);
   // the cells a fine grain may occupy: active and not fully covered by a coarse grain
-  auto open_cell = [&](int c, double &ec) {
-    if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) return false;
-    ec = coarse_eps(a.key[c]);
-    return ec < 1.0;
-  };
-  double denom = 0.0;
-  for (int i = i0; i < i1; ++i)
-    for (int j = j0; j < j1; ++j)
-      for (int k = k0; k < k1; ++k) {
-        double ec;
-        if (!open_cell((i * a.ny + j) * a.nz + k, ec)) continue;
-        const double wb = kernel_mirror(a, xc, yc, zc, i, j, k);
-        if (wb > 0.0) denom += wb;
-      }
+  double ec = 0.0, wb = 0.0;
+  if (b.in && (a.flags[b.c] & LBDEM_FLAG_ACTIVE)) {
+    ec = coarse_eps(a.key[b.c]);
+    if (ec < 1.0) wb = kernel_mirror(a, xc, yc, zc, b.i, b.j, b.k);
+  }
+  const bool w_pos = wb > 0.0;
+  const double denom = wave_sum(w_pos ? wb : 0.0);
   if (denom < 1e-30) return;
-  double dcorr = 0.0;
-  for (int i = i0; i < i1; ++i)
-    for (int j = j0; j < j1; ++j)
-      for (int k = k0; k < k1; ++k) {
-        double ec;
-        if (!open_cell((i * a.ny + j) * a.nz + k, ec)) continue;
-        const double wb = kernel_mirror(a, xc, yc, zc, i, j, k);
-        if (wb > 0.0) dcorr += (wb / denom) * (1.0 - ec);
-      }
-  if (dcorr < 1e-30) return;
+  const double dcorr = wave_sum(w_pos ? (wb / denom) * (1.0 - ec) : 0.0);
+  if (dcorr < 1e-30 || !w_pos) return;
   const double pi = 3.14159265358979323846;
   const double Vp = 4.0 / 3.0 * pi * R * R * R, Vl = a.dx * a.dx * a.dx;
-  const double vx = a.vel[3 * p] * a.dt / a.dx, vy = a.vel[3 * p + 1] * a.dt / a.dx,
-               vz = a.vel[3 * p + 2] * a.dt / a.dx;
-  for (int i = i0; i < i1; ++i)
-    for (int j = j0; j < j1; ++j)
-      for (int k = k0; k < k1; ++k) {
-        const int c = (i * a.ny + j) * a.nz + k;
-        double ec;
-        if (!open_cell(c, ec)) continue;
-        const double wb = kernel_mirror(a, xc, yc, zc, i, j, k);
-        if (wb <= 0.0) continue;
-        const double wn = wb / denom;
-        const double e = wn * (1.0 - ec) / dcorr * Vp / Vl;
-        unsafeAtomicAdd(a.volfrac + c, (float)e);
-        unsafeAtomicAdd(a.velsum + 3 * c, (float)(vx * e));
-        unsafeAtomicAdd(a.velsum + 3 * c + 1, (float)(vy * e));
-        unsafeAtomicAdd(a.velsum + 3 * c + 2, (float)(vz * e));
-        unsafeAtomicAdd(a.wsum + c, (float)wn);
-      }
+  const double wn = wb / denom;
+  const double e = wn * (1.0 - ec) / dcorr * Vp / Vl;
+  const int c = b.c;
+  unsafeAtomicAdd(a.volfrac + c, (float)e);
+  unsafeAtomicAdd(a.velsum + 3 * c, (float)(a.vel[3 * p] * a.dt / a.dx * e));
+  unsafeAtomicAdd(a.velsum + 3 * c + 1, (float)(a.vel[3 * p + 1] * a.dt / a.dx * e));
+  unsafeAtomicAdd(a.velsum + 3 * c + 2, (float)(a.vel[3 * p + 2] * a.dt / a.dx * e));
+  unsafeAtomicAdd(a.wsum + c, (float)wn);
 }
 
 // map_fine_grains, last loop (hybridlattice.py:363-376): clamp to the free fraction, then
@@ -319,39 +316,32 @@ __global__ __launch_bounds__(256) void k_hyb_finalize(const IbmArgs a, int ncell
   }
 }
 
-// _transfer_fine_force + _compute_drag_force (hybridlattice.py:661-740), one thread per grain
-// with 2R <= dx; nodes with a positive kernel weight only; total solid fraction fine + coarse
+// _transfer_fine_force + _compute_drag_force (hybridlattice.py:661-740), one wavefront per
+// grain with 2R <= dx (lane l: cell l of the box, sums as wave reductions); nodes with a
+// positive kernel weight only; total solid fraction fine + coarse
 __global__ __launch_bounds__(256) void k_hyb_drag(const IbmArgs a) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= a.np) return;
+  const int p = blockIdx.x * kGrainsPerBlock + (int)(threadIdx.x >> 6);
+  if (p >= a.np) return;  // uniform over the wavefront
   if (2.0 * a.rad[p] > a.dx) return;
   double xc, yc, zc;
   lattice_pos(a, p, xc, yc, zc);
-  int i0, i1, j0, j1, k0, k1;
-  eq_box(xc, a.nx, i0, i1);
-  eq_box(yc, a.ny, j0, j1);
-  eq_box(zc, a.nz, k0, k1);
-  double vw[3] = {0, 0, 0}, es = 0.0, wt = 0.0;
-  int nl = 0;
-  for (int i = i0; i < i1; ++i)
-    for (int j = j0; j < j1; ++j)
-      for (int k = k0; k < k1; ++k) {
-        const int c = (i * a.ny + j) * a.nz + k;
-        if (!(a.flags[c] & LBDEM_FLAG_ACTIVE)) continue;
-        const double w = kernel_mirror(a, xc, yc, zc, i, j, k);
-        if (w <= 0.0) continue;
+  const BoxLane b = box_lane(a, xc, yc, zc);
+  double w = 0.0;
+  if (b.in && (a.flags[b.c] & LBDEM_FLAG_ACTIVE)) w = kernel_mirror(a, xc, yc, zc, b.i, b.j, b.k);
+  const bool use = w > 0.0;
+  double vw[3];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) vw[m] += (double)a.fvel[3 * c + m] * w;
-        es += 1.0 - (double)a.volfrac[c] - coarse_eps(a.key[c]);
-        wt += w;
-        nl += 1;
-      }
+  for (int m = 0; m < 3; ++m) vw[m] = wave_sum(use ? (double)a.fvel[3 * b.c + m] * w : 0.0);
+  const double es = wave_sum(use ? 1.0 - (double)a.volfrac[b.c] - coarse_eps(a.key[b.c]) : 0.0);
+  const double wt = wave_sum(use ? w : 0.0);
+  const double nl = wave_sum(use ? 1.0 : 0.0);
+  if ((threadIdx.x & 63) != 0) return;
   double uf[3] = {0, 0, 0}, ef = 0.0;
   if (wt > 1e-15) {
 #pragma unroll
     for (int m = 0; m < 3; ++m) uf[m] = vw[m] / wt * a.dx / a.dt;
   }
-  if (nl > 0) ef = es / (double)nl;
+  if (nl > 0.0) ef = es / nl;
   double F[3] = {0, 0, 0};
   const double ep = 1.0 - ef;
   if (ep > 1e-15) {
@@ -654,7 +644,8 @@ int lbdem_hybrid_map_fine(const lbdem_lattice *lat, const uint8_t *flags, const 
   hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, fine_volfrac, (size_t)n);
   hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, velsum, (size_t)(3 * n));
   hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, wsum, (size_t)n);
-  if (g->n > 0) hipLaunchKernelGGL(k_hyb_spread, dim3(nblk(g->n)), dim3(256), 0, s, a);
+  if (g->n > 0)
+    hipLaunchKernelGGL(k_hyb_spread, dim3((g->n + kGrainsPerBlock - 1) / kGrainsPerBlock), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_hyb_finalize, dim3(nblk(n)), dim3(256), 0, s, a, (int)n);
   LBDEM_CHECK_LAUNCH("lbdem_hybrid_map_fine");
   return LBDEM_OK;
@@ -679,7 +670,9 @@ int lbdem_hybrid_forces(const lbdem_lattice *lat, const uint8_t *flags, const lb
   a.fvel = const_cast<float *>(vel);
   a.force_out = g->force_fluid;
   a.moment_out = g->moment_fluid;
-  if (g->n > 0) hipLaunchKernelGGL(k_hyb_drag, dim3(nblk(g->n)), dim3(256), 0, as_stream(stream), a);
+  if (g->n > 0)
+    hipLaunchKernelGGL(k_hyb_drag, dim3((g->n + kGrainsPerBlock - 1) / kGrainsPerBlock), dim3(256), 0,
+                       as_stream(stream), a);
   LBDEM_CHECK_LAUNCH("lbdem_hybrid_forces");
   return LBDEM_OK;
 }
