@@ -1,6 +1,6 @@
 /*
  * ORACLE -- test infrastructure only.  CPU restatement of the reference's lattice algorithms
- * (LBDEM-Taichi, src/lbm3d, src/lbdem, src/unlbdem) used by tests/ as the parity checker and
+ * (LBDEM-Taichi, src/lbm3d, src/lbdem, src/unlbdem, src/hybridlbdem) used by tests/ as the parity checker and
  * by bench.py as the timed CPU baseline.  The product path never links or calls this file.
  *
  * It keeps the reference's structure on purpose (that is what it checks against):
@@ -17,7 +17,16 @@
  *   psc_map / psc_forces: key rule, zero force in fluid at rest;
  *   threedelta, ibm_spread, ibm_drag, ibm_weight, vans_spread, vans_drag: closed-form kernel
  *     values, partition of unity, conservation of spread volume / momentum / force, dilute
- *     (Schiller-Naumann) drag limits (tests/test_oracle_unresolved.py).
+ *     (Schiller-Naumann) drag limits (tests/test_oracle_unresolved.py);
+ *   boundaries: prescribed moments of the rebuilt Zou-He populations, exit equilibrium, channel
+ *     profile (tests/test_oracle_boundaries.py);
+ *   hybrid_map_coarse / _map_fine / _fine_weights / collide_hybrid / _fine_drag: equal to the PSC
+ *     mapping of the coarse grains, conservation of the fine volume and momentum with the
+ *     free-fraction correction, superposition of the single-solid weighted collisions and the
+ *     momentum moved toward each solid, interpolation and dilute drag limit
+ *     (tests/test_oracle_hybrid.py);
+ *   stress / plane_stress: zero at rest, force balance of steady Poiseuille flow
+ *     (tests/test_oracle_stress.py).
  * Parity unpinned: tenneti_cd at finite solid fraction and Reynolds number, and the VANS
  * forcing term F_q of vans_collide -- restatements only, no closed form or reference vector.
  */

@@ -276,3 +276,29 @@ def test_closed_box_conserves_mass():
     assert L.sums()[0] == pytest.approx(m0, rel=1e-13)
     assert np.isfinite(F).all() and np.abs(F[0, :3]).max() > 0 and np.abs(F[1, :3]).max() > 0
     assert (F[1:, 3:] == 0).all()
+
+
+def test_fine_weights_are_the_eqibm_weights_and_respect_the_coarse_room():
+    """Without coarse grains the fine Tenneti weight is the (pinned) EqIBM weight; where fine and
+    coarse fractions leave no fluid the weight keeps its previous value."""
+    g = _mixed()[1:]
+    L = _lattice()
+    L.vel[...] = np.array([0.01, -0.005, 0.002])
+    O.hybrid_map_coarse(L, g, [0, 0, 0], DX, DT)
+    O.hybrid_map_fine(L, _g7(g), [0, 0, 0], DX, DT)
+    O.hybrid_fine_weights(L, DX, DT, RHO)
+    E = O.OracleLattice(*L.CT.shape, OMEGA, kind="eqibm")
+    E.CT[...] = L.CT
+    E.volfrac[...] = L.fine_volfrac
+    E.velsolid[...] = L.fine_velsolid
+    E.vel[...] = L.vel
+    O.ibm_weight(E, DX, DT, RHO)
+    cov = L.fine_volfrac > 0
+    assert cov.sum() > 100
+    np.testing.assert_allclose(L.fine_weight[cov], E.weight[cov], rtol=1e-13)
+    # a cell whose fine and coarse fractions fill it keeps the weight it held
+    c = tuple(np.argwhere(cov)[0])
+    L.volfrac[c] = 1.0 - L.fine_volfrac[c]
+    L.fine_weight[c] = 123.0
+    O.hybrid_fine_weights(L, DX, DT, RHO)
+    assert L.fine_weight[c] == 123.0
