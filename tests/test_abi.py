@@ -79,3 +79,19 @@ def test_header_constants_match_python_binding():
     for name in ("COLL_BGK", "COLL_SMAG_LAGGED", "COLL_SMAG_FIXED", "COUPLE_NONE", "COUPLE_PSC", "COUPLE_EQIBM",
                  "COUPLE_VANS", "COUPLE_HYBRID"):
         assert consts[f"LBDEM_{name}"] == getattr(H, name), name
+
+
+def test_integration_stub_structs_match_the_binding():
+    """The ctypes structs shown in INTEGRATION.md have the binding's (and the header's) layout."""
+    import ctypes as C
+    import src._hip as H
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    ns = {"ctypes": C}
+    for name in ("Lattice", "StepArgs"):
+        m = re.search(rf"^class {name}\(ctypes\.Structure\):\n(?:    .*\n|               .*\n)+", text, flags=re.M)
+        assert m, name
+        exec(m.group(0), ns)
+        got = [(f, C.sizeof(t)) for f, t in ns[name]._fields_]
+        want = [(f, C.sizeof(t)) for f, t in getattr(H, name)._fields_]
+        assert got == want, name
+        assert C.sizeof(ns[name]) == C.sizeof(getattr(H, name))
