@@ -560,6 +560,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
   __shared__ int32_t s_list[kTileBuckets];
   __shared__ int s_nb, s_over;
   __shared__ double s_x[256], s_y[256], s_z[256], s_r[256];
+  __shared__ float s_xf[256], s_yf[256], s_zf[256], s_rf[256], s_mf[256];
   __shared__ int32_t s_j[256];
   if (threadIdx.x < kTileTab) s_tab[threadIdx.x] = -1;
   if (threadIdx.x == 0) s_nb = s_over = 0;
@@ -593,6 +594,13 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
   __syncthreads();
   const int nb = s_nb;
   if (s_over == 0 && nb <= kTileBuckets) {
+    // FP32 pre-test (no periodic axis): a candidate is dropped only when its FP32 distance
+    // exceeds (r_i + r_j)(1 + 1e-6) + 2^-20 (|x_i|_inf + |x_j|_inf), a bound on the FP32
+    // conversion and rounding errors four times over, so every pair the FP64 test could call
+    // touching reaches it
+    const bool pre = !(P.periodic[0] || P.periodic[1] || P.periodic[2]);
+    const float xif = (float)xi[0], yif = (float)xi[1], zif = (float)xi[2], rif = (float)ri;
+    const float mif = fmaxf(fabsf(xif), fmaxf(fabsf(yif), fabsf(zif)));
     for (int u = 0; u < nb; ++u) {
       const int32_t bk = s_list[u];
       bool mine = false;
@@ -604,11 +612,18 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
         const int t = base + (int)threadIdx.x;
         if (t < e) {
           const int32_t j = pid[t];
+          const double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2], r = rad[j];
           s_j[threadIdx.x] = j;
-          s_x[threadIdx.x] = pos[3 * j];
-          s_y[threadIdx.x] = pos[3 * j + 1];
-          s_z[threadIdx.x] = pos[3 * j + 2];
-          s_r[threadIdx.x] = rad[j];
+          s_x[threadIdx.x] = x;
+          s_y[threadIdx.x] = y;
+          s_z[threadIdx.x] = z;
+          s_r[threadIdx.x] = r;
+          const float xf = (float)x, yf = (float)y, zf = (float)z;
+          s_xf[threadIdx.x] = xf;
+          s_yf[threadIdx.x] = yf;
+          s_zf[threadIdx.x] = zf;
+          s_rf[threadIdx.x] = (float)r;
+          s_mf[threadIdx.x] = fmaxf(fabsf(xf), fmaxf(fabsf(yf), fabsf(zf)));
         }
         __syncthreads();
         if (active && mine) {
@@ -616,6 +631,12 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
           for (int q = 0; q < m; ++q) {
             const int32_t j = s_j[q];
             if (j <= i) continue;
+            if (pre) {
+              const float dx = xif - s_xf[q], dy = yif - s_yf[q], dz = zif - s_zf[q];
+              const float x2 = dx * dx + dy * dy + dz * dz;
+              const float T = (rif + s_rf[q]) * 1.000001f + 9.53674316e-7f * (mif + s_mf[q]);
+              if (x2 > T * T * 1.000004f) continue;
+            }
             const double xj[3] = {s_x[q], s_y[q], s_z[q]};
             if (touching(P, xi, ri, xj, s_r[q])) emit_pair(i, j, pairs, cap, npairs);
           }
