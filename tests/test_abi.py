@@ -82,7 +82,7 @@ def test_header_constants_match_python_binding():
 
 
 def test_integration_stub_structs_match_the_binding():
-    """The ctypes structs shown in INTEGRATION.md have the binding's (and the header's) layout."""
+    """The ctypes structs shown in the integration guide have the binding's (and the header's) layout."""
     import ctypes as C
     import src._hip as H
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()

@@ -1,4 +1,4 @@
-"""The ctypes binding shown in INTEGRATION.md, run as written, against BasicLattice3D."""
+"""The ctypes binding shown in the integration guide, run as written, against BasicLattice3D."""
 import os
 import re
 
@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 def _stub():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     m = re.search(r"```python\n(import ctypes, math, torch\n.*?)```", text, flags=re.S)
-    assert m, "the ctypes stub is missing from INTEGRATION.md"
+    assert m, "the ctypes stub is missing from the integration guide"
     return m.group(1)
 
 
