@@ -560,7 +560,8 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
   __shared__ int32_t s_list[kTileBuckets];
   __shared__ int s_nb, s_over;
   __shared__ double s_x[256], s_y[256], s_z[256], s_r[256];
-  __shared__ float s_xf[256], s_yf[256], s_zf[256], s_rf[256], s_mf[256];
+  __shared__ float4 s_pf[256];  // FP32 x, y, z, r of the staged candidates (one LDS read each)
+  __shared__ float s_mf[256];
   __shared__ int32_t s_j[256];
   if (threadIdx.x < kTileTab) s_tab[threadIdx.x] = -1;
   if (threadIdx.x == 0) s_nb = s_over = 0;
@@ -619,10 +620,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
           s_z[threadIdx.x] = z;
           s_r[threadIdx.x] = r;
           const float xf = (float)x, yf = (float)y, zf = (float)z;
-          s_xf[threadIdx.x] = xf;
-          s_yf[threadIdx.x] = yf;
-          s_zf[threadIdx.x] = zf;
-          s_rf[threadIdx.x] = (float)r;
+          s_pf[threadIdx.x] = make_float4(xf, yf, zf, (float)r);
           s_mf[threadIdx.x] = fmaxf(fabsf(xf), fmaxf(fabsf(yf), fabsf(zf)));
         }
         __syncthreads();
@@ -632,9 +630,10 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
             const int32_t j = s_j[q];
             if (j <= i) continue;
             if (pre) {
-              const float dx = xif - s_xf[q], dy = yif - s_yf[q], dz = zif - s_zf[q];
+              const float4 pj = s_pf[q];
+              const float dx = xif - pj.x, dy = yif - pj.y, dz = zif - pj.z;
               const float x2 = dx * dx + dy * dy + dz * dz;
-              const float T = (rif + s_rf[q]) * 1.000001f + 9.53674316e-7f * (mif + s_mf[q]);
+              const float T = (rif + pj.w) * 1.000001f + 9.53674316e-7f * (mif + s_mf[q]);
               if (x2 > T * T * 1.000004f) continue;
             }
             const double xj[3] = {s_x[q], s_y[q], s_z[q]};
