@@ -43,6 +43,11 @@ static void cell_of(const double *x, const double *dmin, double cs, int32_t *ijk
 
 uint32_t lbo_morton3d32(int32_t x, int32_t y, int32_t z) { return morton(x, y, z); }
 
+static int cmp_i32(const void *a, const void *b) {
+  const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+  return (x > y) - (x < y);
+}
+
 /* BPCD.create table size, bpcd.py:48-65, with next_pow2 (bpcd/utils.py:12-19) */
 static int64_t next_pow2(int64_t x) {
   x -= 1;
@@ -69,9 +74,19 @@ int64_t lbo_hash_table_size(const double *dmin, const double *dmax, double cell)
  * (capacity cap pairs); returns the number of distinct pairs (may exceed cap). */
 int64_t lbo_broadphase(int32_t n, const double *pos, const double *dmin, double cs, int32_t T, int32_t probe27,
                        int32_t *pairs, int64_t cap) {
-  int32_t *count = calloc((size_t)T, sizeof(int32_t));
-  int32_t *offset = calloc((size_t)T, sizeof(int32_t));
-  int32_t *current = calloc((size_t)T, sizeof(int32_t));
+  /* the three table arrays persist between calls (a table of 2^20..2^22 entries is zeroed
+   * once; afterwards only the buckets this call touched are cleared again).  Not reentrant. */
+  static int32_t *count = NULL, *offset = NULL, *current = NULL;
+  static int32_t cap_T = 0;
+  if (T > cap_T) {
+    free(count);
+    free(offset);
+    free(current);
+    count = calloc((size_t)T, sizeof(int32_t));
+    offset = calloc((size_t)T, sizeof(int32_t));
+    current = calloc((size_t)T, sizeof(int32_t));
+    cap_T = T;
+  }
   int32_t *pid = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   int32_t *hsh = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   for (int32_t i = 0; i < n; ++i) {
@@ -80,12 +95,19 @@ int64_t lbo_broadphase(int32_t n, const double *pos, const double *dmin, double 
     hsh[i] = hash_code(ijk[0], ijk[1], ijk[2], T);
     count[hsh[i]] += 1;
   }
-  /* exclusive scan (the reference uses a Blelloch scan; any exact scan gives the same) */
+  /* exclusive scan (the reference uses a Blelloch scan; any exact scan gives the same).  Only
+   * the occupied buckets are visited, in increasing hash order: their offsets equal those of
+   * the dense scan, and an empty bucket's offset is never read (its count is 0). */
+  int32_t *occ = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  memcpy(occ, hsh, sizeof(int32_t) * (size_t)n);
+  qsort(occ, (size_t)n, sizeof(int32_t), cmp_i32);
   int32_t run = 0;
-  for (int32_t h = 0; h < T; ++h) {
-    offset[h] = run;
-    run += count[h];
+  for (int32_t u = 0; u < n; ++u) {
+    if (u > 0 && occ[u] == occ[u - 1]) continue;
+    offset[occ[u]] = run;
+    run += count[occ[u]];
   }
+  free(occ);
   for (int32_t i = 0; i < n; ++i) pid[offset[hsh[i]] + current[hsh[i]]++] = i;
   int64_t np = 0;
   for (int32_t i = 0; i < n; ++i) {
@@ -135,9 +157,7 @@ int64_t lbo_broadphase(int32_t n, const double *pos, const double *dmin, double 
       }
     }
   }
-  free(count);
-  free(offset);
-  free(current);
+  for (int32_t i = 0; i < n; ++i) count[hsh[i]] = current[hsh[i]] = offset[hsh[i]] = 0;
   free(pid);
   free(hsh);
   return np;

@@ -491,3 +491,59 @@ class OracleDEM:
 
     def __getitem__(self, k):
         return self.a[k]
+
+
+# ---------------------------------------------------------------------------------------
+# single-sphere VANS settling, example/unlbdem/single_particle_settle/one_particle.py
+# ---------------------------------------------------------------------------------------
+# the four cases of one_particle.py:44-73: Re -> (mu [Pa s], rho [kg/m^3]); sphere of 1120 kg/m^3
+SETTLE_CASES = {"31.9": (58e-3, 960.0), "11.6": (113e-3, 962.0), "4.1": (212e-3, 965.0), "1.5": (373e-3, 970.0)}
+SETTLE_DX, SETTLE_TAU, SETTLE_R, SETTLE_DENS = 0.025, 0.5001, 0.0075, 1120.0
+SETTLE_MASS = 0.00197920337176   # one_particle.p4p
+SETTLE_X0 = (0.05, 0.1275, 0.05)
+
+
+def settle_setup(case):
+    """(mu, rho, dt, (nx, ny, nz), gravity) of one_particle.py:76-110 for a case."""
+    mu, rho = SETTLE_CASES[case]
+    nu_lu = (SETTLE_TAU - 0.5) / 3.0
+    dt = SETTLE_DX ** 2 / ((mu / rho) / nu_lu)
+    shape = (int(0.1 / SETTLE_DX) + 1, int(0.16 / SETTLE_DX) + 1, int(0.1 / SETTLE_DX) + 1)
+    g = (0.0, -9.81 * (SETTLE_DENS - rho) / SETTLE_DENS, 0.0)
+    return mu, rho, dt, shape, g
+
+
+def settle_container(CT):
+    """setContainer (one_particle.py:34-40): every face cell is an obstacle."""
+    CT[0], CT[-1], CT[:, 0], CT[:, -1], CT[:, :, 0], CT[:, :, -1] = (OBSTACLE,) * 6
+
+
+class VansSettleOracle:
+    """one_particle.py's main loop (:236-246) on the restated VANS lattice and DEM: per lattice
+    step grains2lattice, compute_hydrodynamic_force, 10 DEM substeps, force2lattice, collide,
+    stream.  after_grains2lattice(L, pos) is called after the volume-fraction mapping, before the
+    drag is evaluated."""
+
+    def __init__(self, case, dem_cfg, after_grains2lattice=None):
+        self.mu, self.rho, self.dt, shape, _ = settle_setup(case)
+        self.L = OracleLattice(*shape, 1.0 / SETTLE_TAU, kind="vans")
+        settle_container(self.L.CT)
+        self.dem = OracleDEM(dem_cfg, [SETTLE_R], [SETTLE_MASS], [list(SETTLE_X0)])
+        self.hook = after_grains2lattice
+        self.L.initialize()
+        self.t = 0
+
+    def step(self):
+        L, a, dx, R = self.L, self.dem.a, SETTLE_DX, SETTLE_R
+        vans_spread(L, np.column_stack([a["pos"], [[R]], np.zeros((1, 3))]), [0, 0, 0], dx, self.dt, self.rho,
+                    with_force=False)
+        if self.hook is not None:
+            self.hook(L, a["pos"][0].copy())
+        a["force_fluid"][...] = vans_drag(L, np.column_stack([a["pos"], a["vel"], [[R]]]), [0, 0, 0], dx, self.dt,
+                                          self.rho)
+        self.dem.step(10)
+        vans_spread(L, np.column_stack([a["pos"], [[R]], a["force_fluid"]]), [0, 0, 0], dx, self.dt, self.rho,
+                    with_force=True)
+        L.collide()
+        L.stream()
+        self.t += 1
