@@ -1,20 +1,30 @@
-"""Benchmark: resolved PSC LBM-DEM coupled time step on one MI355X.
+"""Benchmark: the fused D3Q19 collide-stream on one MI355X, on the 256^3 Taylor-Green vortex.
 
-Workload (the configuration the headline metric is quoted on): 128 x 128 x 256 D3Q19 lattice,
-FP32 populations, tau = 0.55, bounce-back walls on all faces; 200 spheres of radius 8 lattice
-units at seeded non-overlapping uniform positions, density ratio 2.5, gravity -1e-5 lu along z,
-Hertz-Mindlin contacts (E = 1e5 lu, nu = 0.3, e = 0.9, friction 0.3), 10 DEM substeps per
-lattice step.  A step is: grain -> lattice mapping, fused PSC collide-stream, FP64 grain force
-reduction, 10 DEM substeps.  Value: million lattice-cell updates per second of the whole step.
+Headline workload: 256^3 periodic Taylor-Green vortex, Re 1600, u0 = 0.05 lu (tau = 0.50382),
+BGK + Smagorinsky (Cs = 0.1, lag-free fixed point), FP32 populations (SoA, AA in-place), one
+fused collide-stream launch per step.  Value: million lattice-cell updates per second of the
+timed steps (inputs resident in HBM).
 
-Prints one JSON line.  `roofline` is for the collide-stream kernel (HIP events around each of
-its launches in the timed region): algorithmic bytes = 152 B per active cell (19 FP32 loads +
-19 stores).  `cpu_baseline` times the oracle's restatement of the reference's two-pass
-algorithm (FP32, OpenMP) on the host cores over a bounded sample of the same workload.
+The JSON line also carries:
+  roofline      -- the collide-stream kernel: 152 B per cell (19 FP32 loads + 19 stores) over
+                   its average launch time from HIP events around sampled launches in the timed
+                   region; traffic = HBM bytes per launch from the committed rocprofv3 counter
+                   summary (FETCH_SIZE, WRITE_SIZE, copy-calibrated);
+  cpu_baseline  -- the oracle's restatement of the reference's two-pass algorithm (lagged
+                   Smagorinsky, AoS, FP32, OpenMP on the host cores) on the same vortex, over a
+                   bounded number of steps;
+  extras        -- the reference's lagged Smagorinsky on the same vortex; FP16 / BF16 shifted
+                   storage with its speed ratio and its velocity error against the FP64 run after
+                   1,000 steps; the lagged vs lag-free velocity difference; the 64^3 lid-driven
+                   cavity (2,000 steps) on the GPU and on the CPU restatement; the resolved PSC
+                   LBM-DEM coupled step (128x128x256, 200 grains of radius 8) with its
+                   collide-stream roofline; the adjoint / forward time ratio of the 96^3
+                   differentiable PSC channel.
 """
 import argparse
 import gc
 import json
+import math
 import os
 import sys
 import time
@@ -24,8 +34,195 @@ sys.path.insert(0, os.path.join(ROOT, "lbdem-taichi_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TGV_N, TGV_U0, TGV_RE, TGV_CS = 256, 0.05, 1600.0, 0.1
 NX, NY, NZ = 128, 128, 256
 NGRAIN, RADIUS, TAU, SUBCYCLES = 200, 8.0, 0.55, 10
+
+
+def host_cpu():
+    """CPU model name and the threads this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def tgv_fields(n, u0=TGV_U0, Re=TGV_RE):
+    """Velocity and density of the Taylor-Green vortex and the tau giving Re = u0 / (k nu)."""
+    import numpy as np
+    k = 2 * np.pi / n
+    x = np.arange(n, dtype=np.float32)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    v = np.zeros((n, n, n, 3), np.float32)
+    v[..., 0] = u0 * np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z)
+    v[..., 1] = -u0 * np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z)
+    rho = (1.0 + 3.0 * u0 * u0 / 16.0 * (np.cos(2 * k * X) + np.cos(2 * k * Y)) * (np.cos(2 * k * Z) + 2))
+    return v, rho.astype(np.float32), 3.0 * u0 / (k * Re) + 0.5
+
+
+def tgv_lattice(storage, smag="fixed", n=TGV_N, kernel=0):
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    BasicLattice3D.SmagorinskyConstant = TGV_CS
+    v, rho, tau = tgv_fields(n)
+    L = BasicLattice3D(n, n, n, 1.0 / tau, 1.0, 1.0, 1.0, storage=storage, smagorinsky=smag)
+    L.kernel = kernel
+    L.vel.from_numpy(v)
+    L.rho.from_numpy(rho)
+    L.initialize()
+    return L
+
+
+def time_steps(L, steps, warmup, sample_every=4):
+    """Whole-step time over `steps` steps (events bracketing the loop after a synchronize) and
+    the collide-stream kernel time from event pairs around two consecutive launches (one even,
+    one odd) in every `sample_every`."""
+    import torch
+    for _ in range(warmup):
+        L.step()
+    torch.cuda.synchronize()
+    ev = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for s in range(steps):
+        L.step_events = ev if s % sample_every < 2 else None
+        L.step()
+    e1.record()
+    torch.cuda.synchronize()
+    L.step_events = None
+    ms = e0.elapsed_time(e1) / steps
+    # launches alternate parity; the first sampled launch of each pair has the parity of its step
+    par = {0: [], 1: []}
+    t = L.t - steps
+    k = 0
+    for s in range(steps):
+        if s % sample_every < 2:
+            a, b = ev[k]
+            par[(t + s) % 2].append(a.elapsed_time(b))
+            k += 1
+    kms = {p: sum(v) / len(v) for p, v in par.items() if v}
+    return ms, kms
+
+
+def tgv_entry(storage, smag, steps, warmup, n=TGV_N, kernel=0):
+    import torch
+    L = tgv_lattice(storage, smag, n, kernel)
+    ms, kms = time_steps(L, steps, warmup)
+    del L
+    torch.cuda.empty_cache()
+    k_ms = sum(kms.values()) / len(kms)
+    bpc = {"f32": 152, "bf16": 76, "f16": 76, "f64": 304}[storage]
+    out = {"mlups": round(n ** 3 / ms / 1e3, 1), "ms_per_step": round(ms, 5), "kernel_ms": round(k_ms, 5),
+           "kernel_ms_even_odd": [round(kms.get(0, 0.0), 5), round(kms.get(1, 0.0), 5)],
+           "bytes_per_cell": bpc, "achieved_GBs": round(n ** 3 * bpc / k_ms / 1e6, 1)}
+    out["frac"] = round(out["achieved_GBs"] / HBM_PEAK_GBS, 4)
+    if smag == "lagged":  # + rho, u and omega of the previous collision, read and written
+        out["bytes_per_cell_with_lag_state"] = bpc + (80 if storage == "f64" else 40)
+        out["frac_with_lag_state"] = round(n ** 3 * out["bytes_per_cell_with_lag_state"] / k_ms / 1e6 / HBM_PEAK_GBS, 4)
+    return out
+
+
+def tgv_errors(steps=1000, n=TGV_N):
+    """Relative L2 velocity error of each storage against the FP64 run after `steps` steps (both
+    Smagorinsky forms), and the lagged vs lag-free difference of the FP64 runs."""
+    import torch
+    ref, out = {}, {}
+    for smag in ("fixed", "lagged"):
+        for storage in ("f64", "f32", "f16", "bf16"):
+            L = tgv_lattice(storage, smag, n)
+            L.step(steps)
+            L._refresh_macros()
+            u = L.vel.dev.to(torch.float64)
+            if storage == "f64":
+                ref[smag] = u.clone()
+            else:
+                out[f"{storage}_{smag}"] = float(torch.linalg.vector_norm(u - ref[smag]) /
+                                                 torch.linalg.vector_norm(ref[smag]))
+            del L, u
+            torch.cuda.empty_cache()
+    out["lagged_vs_fixed_f64"] = float(torch.linalg.vector_norm(ref["lagged"] - ref["fixed"]) /
+                                       torch.linalg.vector_norm(ref["lagged"]))
+    return {k: float(f"{v:.4g}") for k, v in out.items()}
+
+
+def cpu_baseline_tgv(budget_s=15.0, n=TGV_N):
+    """The oracle's FP32 two-pass restatement of the reference (lagged Smagorinsky) on the same
+    vortex, all host threads, as many steps as fit in budget_s after one warm-up step."""
+    import numpy as np
+    from oracle import oracle as O
+    model, threads = host_cpu()
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    v, rho, tau = tgv_fields(n)
+    orc = O.OracleLattice(n, n, n, 1.0 / tau, kind="basic", cs=TGV_CS, real=np.float32)
+    orc.vel[...] = v
+    orc.rho[...] = rho
+    del v, rho
+    orc.initialize()
+    orc.step()
+    k, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and k < 100:
+        orc.step()
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(n ** 3 * k / dt / 1e6, 2), "unit": "MLUPS", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "port", "cpu_model": model,
+            "sample": f"{k} steps of the same {n}^3 vortex after 1 warm-up step: the oracle's C restatement of "
+                      "the reference's two-pass step (collide into fpc with lagged Smagorinsky, pull-stream "
+                      "with per-neighbour cell-type dispatch, AoS), FP32, OpenMP"}
+
+
+def cavity_c1(steps=2000, n=64, cpu=True):
+    """64^3 lid-driven cavity (walls on five faces, lid u = (0.1, 0, 0) on +z), tau 0.6,
+    lagged Smagorinsky Cs 0.17, FP32, 2,000 steps: GPU (median of 3 runs) and the oracle's
+    restatement on the host (one run after 50 warm-up steps), and the velocity difference."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    BasicLattice3D.SmagorinskyConstant = 0.17
+    orc = O.OracleLattice(n, n, n, 1.0 / 0.6, kind="basic", cs=0.17, real=np.float32)
+    O.cavity(orc, (0.1, 0.0, 0.0), lid_axis=2)
+    runs, L = [], None
+    for _ in range(3):
+        L = BasicLattice3D(n, n, n, 1.0 / 0.6, 1.0, 1.0, 1.0, storage="f32", smagorinsky="lagged")
+        L.CT.from_numpy(orc.CT)
+        L.vel.from_numpy(orc.vel.astype(np.float32))
+        L.initialize()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        L.step(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        runs.append(e0.elapsed_time(e1))
+    gpu_ms = sorted(runs)[1]
+    out = {"lattice": n, "steps": steps, "gpu_s": round(gpu_ms / 1e3, 4),
+           "gpu_mlups": round(n ** 3 * steps / gpu_ms / 1e3, 1)}
+    if cpu:
+        model, threads = host_cpu()
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        warm = O.OracleLattice(n, n, n, 1.0 / 0.6, kind="basic", cs=0.17, real=np.float32)
+        O.cavity(warm, (0.1, 0.0, 0.0), lid_axis=2)
+        warm.initialize()
+        warm.step(50)
+        del warm
+        orc.initialize()
+        t0 = time.perf_counter()
+        orc.step(steps)
+        cs = time.perf_counter() - t0
+        fluid = orc.CT == O.FLUID
+        u = L.vel.to_numpy()[fluid].astype(np.float64)
+        r = orc.vel[fluid].astype(np.float64)
+        out.update(cpu_s=round(cs, 2), cpu_mlups=round(n ** 3 * steps / cs / 1e6, 2),
+                   cpu_cores=int(os.environ["OMP_NUM_THREADS"]), cpu_model=model,
+                   gpu_over_cpu=round(cs * 1e3 / gpu_ms, 1),
+                   velocity_rel_l2_gpu_vs_cpu=float(f"{np.linalg.norm(u - r) / np.linalg.norm(r):.3g}"))
+    return out
 
 
 def grain_layout(seed):
@@ -65,7 +262,7 @@ def walls_ct(nx, ny, nz, dtype="int32"):
     return ct
 
 
-def build_gpu(seed, storage="f32"):
+def build_psc(seed, storage="f32"):
     from src.dem3d.demsolver import DEMSolver
     from src.lbdem.psclattice3d import PSCLattice3D
     pos, rad, mass, L = grain_layout(seed)
@@ -85,43 +282,86 @@ def coupled_step(lat, dem):
     dem.run_substeps(SUBCYCLES)
 
 
-def cpu_baseline(seed, budget_s=20.0):
-    """Oracle restatement (reference algorithm, FP32, OpenMP) of the same coupled step."""
+def coupled_c2(seed=2026, steps=100, warmup=10):
+    """Resolved PSC LBM-DEM: 128 x 128 x 256 FP32 lattice, tau 0.55, walls on all faces; 200
+    spheres of radius 8 lu (seeded, non-overlapping), density ratio 2.5, gravity -1e-5 lu along
+    z, Hertz-Mindlin (E = 1e5 lu, nu = 0.3, e = 0.9, friction 0.3), 10 DEM substeps per lattice
+    step.  A step: grain -> lattice mapping, fused PSC collide-stream, FP64 grain force
+    reduction, 10 DEM substeps.  Roofline of the PSC collide-stream: 152 B per active cell."""
+    import torch
+    lat, dem = build_psc(seed)
+    for _ in range(warmup):
+        coupled_step(lat, dem)
+    torch.cuda.synchronize()
+    ev = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for s in range(steps):
+        lat.step_events = ev if s % 16 < 2 else None
+        coupled_step(lat, dem)
+    e1.record()
+    torch.cuda.synchronize()
+    lat.step_events = None
+    ms = e0.elapsed_time(e1) / steps
+    k_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+    dem.check()
+    alg = 152 * int(lat._active_np.sum())
+    out = {"lattice": [NX, NY, NZ], "grains": NGRAIN, "seed": seed, "dem_substeps": SUBCYCLES,
+           "ms_per_step": round(ms, 4), "mlups": round(NX * NY * NZ / ms / 1e3, 1),
+           "collide_stream_kernel_ms": round(k_ms, 4), "collide_stream_bytes_per_launch": alg,
+           "collide_stream_achieved_GBs": round(alg / k_ms / 1e6, 1),
+           "collide_stream_frac": round(alg / k_ms / 1e6 / HBM_PEAK_GBS, 4)}
+    tfile = os.path.join(ROOT, "profiles", "psc_collide_stream_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            out["collide_stream_traffic_from_profile"] = json.load(fh).get("bytes_per_launch")
+    del lat, dem
+    torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_c5(n=96, steps=2000, K=100, nsph=32, R=6.0, seed=7):
+    """Adjoint / forward time of the 96^3 differentiable PSC channel (walls at y = 0 and n - 1,
+    periodic x and z, F_x = 1e-6, 32 fixed spheres of radius 6, tau 0.6, 2,000 steps,
+    checkpoints every 100): forward rollout time and backward (recompute + adjoint) time."""
     import numpy as np
-    from oracle import oracle as O
-    threads = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    pos, rad, mass, L = grain_layout(seed)
-    cfg = dem_config(L)
-    orc = O.OracleLattice(NX, NY, NZ, 1.0 / TAU, kind="psc", real=np.float32)
-    orc.CT[...] = walls_ct(NX, NY, NZ)
-    dem = O.OracleDEM(cfg, rad, mass, pos, None)
-    orc.initialize()
-
-    def grains():
-        a = dem.a
-        return np.column_stack([a["pos"], a["vel"], a["omega"], a["radius"]])
-
-    def step():
-        g = grains()
-        O.psc_map(orc, g, [0, 0, 0], 1.0, 1.0)
-        orc.collide()
-        F = O.psc_forces(orc, g, [0, 0, 0], 1.0, 1.0, 1.0)
-        dem.a["force_fluid"][...] = F[:, :3]
-        dem.a["moment_fluid"][...] = F[:, 3:]
-        orc.stream()
-        dem.step(SUBCYCLES)
-
-    step()  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s and n < 50:
-        step()
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": NX * NY * NZ * n / dt / 1e6, "unit": "MLUPS", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": "port",
-            "sample": f"{n} coupled steps of the same 128x128x256 / 200-grain workload (PSC map over every "
-                      f"cell x grain, weighted collide, stream, FP64 forces, 10 DEM substeps), FP32, OpenMP"}
+    import torch
+    from src.adjoint import DiffPSCChannel, PSCRollout, sphere_fraction
+    from src.lbm3d.lbmutils import CellType
+    rng = np.random.default_rng(seed)
+    CT = np.full((n, n, n), CellType.FLUID, dtype=np.int32)
+    CT[:, 0, :] = CellType.OBSTACLE
+    CT[:, -1, :] = CellType.OBSTACLE
+    cen = []
+    while len(cen) < nsph:
+        c = np.array([rng.uniform(R + 1, n - R - 2), rng.uniform(R + 2, n - R - 3), rng.uniform(R + 1, n - R - 2)])
+        if all(np.linalg.norm(c - d) > 2 * R + 1 for d in cen):
+            cen.append(c)
+    eps0 = sphere_fraction((n, n, n), cen, R)
+    out = {"lattice": n, "steps": steps, "checkpoint_every": K, "spheres": nsph}
+    for prec in (32, 64):
+        dt = torch.float64 if prec == 64 else torch.float32
+        best = None
+        for _ in range(2):   # the first rollout loads the kernels
+            ch = DiffPSCChannel(CT, precision=prec, K=K)
+            F = torch.tensor([1e-6, 0.0, 0.0], dtype=dt, device="cuda", requires_grad=True)
+            tau = torch.tensor(0.6, dtype=dt, device="cuda", requires_grad=True)
+            e = torch.tensor(eps0, dtype=dt, device="cuda", requires_grad=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            u = PSCRollout.apply(F, tau, e, ch, steps)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            (u.double() ** 2).mean().backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            best = (t1 - t0, t2 - t1)
+            del ch, u
+        out[f"fp{prec}"] = {"forward_ms_per_step": round(best[0] / steps * 1e3, 4),
+                            "adjoint_ms_per_step": round(best[1] / steps * 1e3, 4),
+                            "adjoint_over_forward": round(best[1] / best[0], 3)}
+    torch.cuda.empty_cache()
+    return out
 
 
 def channel_study(nx=256, ny=128, nz=128, steps=100):
@@ -269,40 +509,11 @@ def tgv_variants_study(n=256):
              ("bf16", "fixed", 3), ("f16", "fixed", 4), ("f16", "fixed", 3), ("f64", "fixed", 3)]
     out = []
     for storage, smag, kernel in cases:
-        r = tgv_extra(storage, smag=smag, n=n, kernel=kernel)
+        r = tgv_entry(storage, smag, 20, 4, n=n, kernel=kernel)
         r.update(storage=storage, smagorinsky=smag, kernel=kernel)
         print(f"tgv {storage} {smag} kernel {kernel}: {r['mlups']} MLUPS", file=sys.stderr, flush=True)
         out.append(r)
     return {"study": "tgv", "lattice": n, "cases": out}
-
-
-def tgv_extra(storage, smag="fixed", n=256, steps=20, kernel=0):
-    """Collide-stream MLUPS and HBM fraction on the 256^3 periodic Taylor-Green vortex."""
-    import numpy as np
-    import torch
-    from src.lbm3d.lbm_solver3d import BasicLattice3D
-    k = 2 * np.pi / n
-    x = np.arange(n)
-    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
-    v = np.zeros((n, n, n, 3), np.float32)
-    v[..., 0] = 0.05 * np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z)
-    v[..., 1] = -0.05 * np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z)
-    Lt = BasicLattice3D(n, n, n, 1.0 / 0.50382, 1.0, 1.0, 1.0, storage=storage, smagorinsky=smag)
-    Lt.kernel = kernel
-    Lt.vel.from_numpy(v)
-    Lt.initialize()
-    Lt.step(4)
-    torch.cuda.synchronize()
-    Lt.step_events = []
-    Lt.step(steps)
-    torch.cuda.synchronize()
-    ms = sum(a.elapsed_time(b) for a, b in Lt.step_events) / len(Lt.step_events)
-    bpc = {"f32": 152, "bf16": 76, "f16": 76, "f64": 304}[storage] + (40 if smag == "lagged" else 0)
-    gbs = n ** 3 * bpc / ms / 1e6
-    del Lt
-    torch.cuda.empty_cache()
-    return {"mlups": round(n ** 3 / ms / 1e3, 1), "achieved_GBs": round(gbs, 1),
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": bpc}
 
 
 def precision_study(n=256, steps=20000, every=500, checkpoints=(1000, 5000, 10000, 20000), smag="lagged"):
@@ -526,41 +737,29 @@ def adjoint_study(n=96, steps=2000, K=100, nsph=32, R=6.0, seed=7, n_fd_cells=16
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv", "channel", "hybrid"],
+    ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv", "channel", "hybrid", "coupled"],
                     default=None,
                     help="run a study instead of the benchmark and print its JSON summary")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 512 MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the coupled step is built for a single MI355X (got --gpus {args.gpus}); "
-              "multi-GPU sharding is a design note only", file=sys.stderr)
+        print(f"bench.py measures one GPU (got --gpus {args.gpus}): every path here runs on a single MI355X, and "
+              "sharding over several GPUs is a design note only", file=sys.stderr)
         sys.exit(2)
 
     import torch
     torch.cuda.set_device(0)
-    if args.study == "precision":
-        print(json.dumps(precision_study()), flush=True)
-        return
-    if args.study == "adjoint":
-        print(json.dumps(adjoint_study()), flush=True)
-        return
-    if args.study == "unresolved":
-        print(json.dumps(unresolved_study()), flush=True)
-        return
-    if args.study == "tgv":
-        print(json.dumps(tgv_variants_study()), flush=True)
-        return
-    if args.study == "channel":
-        print(json.dumps(channel_study()), flush=True)
-        return
-    if args.study == "hybrid":
-        print(json.dumps(hybrid_study()), flush=True)
+    studies = {"precision": precision_study, "adjoint": adjoint_study, "unresolved": unresolved_study,
+               "tgv": tgv_variants_study, "channel": channel_study, "hybrid": hybrid_study,
+               "coupled": lambda: coupled_c2(args.seed, args.steps, args.warmup)}
+    if args.study:
+        print(json.dumps(studies[args.study]()), flush=True)
         return
     if args.calibrate_copy:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -569,71 +768,63 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    lat, dem = build_gpu(args.seed)
-    # Python's cyclic garbage collector is off in the timed region (as in timeit): a full
+
+    n = TGV_N
+    # Python's cyclic garbage collector is off in the timed regions (as in timeit): a full
     # collection of the interpreter's heap stalls the host for tens of milliseconds
     gc.collect()
     gc.disable()
-    for _ in range(args.warmup):
-        coupled_step(lat, dem)
-    torch.cuda.synchronize()
-    # Kernel time of the collide-stream launches from HIP events on two consecutive steps (one
-    # even, one odd) in every 16: an event pair around a launch delays it by a few microseconds.
-    ev = []
-    t0 = time.perf_counter()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for s in range(args.steps):
-        lat.step_events = ev if s % 16 < 2 else None
-        coupled_step(lat, dem)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    t_wall = time.perf_counter()
+    head = tgv_entry("f32", "fixed", args.steps, args.warmup)
+    wall = time.perf_counter() - t_wall
     gc.enable()
-    ms_step = e0.elapsed_time(e1) / args.steps
-    lat.step_events = None
-    k_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
-    dem.check()
-    n_active = int(lat._active_np.sum())
-    alg_bytes = 152 * n_active
-    achieved = alg_bytes / k_ms / 1e6
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "psc_collide_stream_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
+    traffic, tsrc = None, os.path.join("profiles", "tgv256_collide_stream_traffic.json")
+    if os.path.exists(os.path.join(ROOT, tsrc)):
+        with open(os.path.join(ROOT, tsrc)) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
-
-    extras = {}
-    if not args.no_extras:
-        for storage in ("f32", "bf16", "f16"):
-            extras[f"tgv256_{storage}"] = tgv_extra(storage)
-        extras["tgv256_bf16_over_f32"] = round(extras["tgv256_bf16"]["mlups"] / extras["tgv256_f32"]["mlups"], 3)
     out = {
         "metric": "MLUPS & %HBM peak (FP32 vs BF16-store @stated err); adjoint/forward ratio",
-        "value": round(NX * NY * NZ / ms_step / 1e3, 1),
+        "value": head["mlups"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded sphere packing)",
-        "config": {"workload": "resolved PSC LBM-DEM, 128x128x256 D3Q19, 200 spheres R=8 lu, tau=0.55, "
-                               "Hertz E=1e5 lu, 10 DEM substeps/step", "lattice": [NX, NY, NZ],
-                   "grains": NGRAIN, "seed": args.seed, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "fused PSC collide-stream, row-tiled, one cell per thread "
-                               "(k_step_pencil<float, BGK, PSC, V=1>)",
-                     "kernel_ms": round(k_ms, 4), "bytes_per_launch": alg_bytes},
+        "data": "synthetic (Taylor-Green vortex initial field)",
+        "config": {"workload": f"{n}^3 periodic Taylor-Green vortex, Re 1600, u0 0.05 lu, D3Q19 BGK + Smagorinsky "
+                               "Cs 0.1 (lag-free fixed point), FP32 populations, one fused collide-stream per step",
+                   "lattice": [n, n, n], "tau": round(3.0 * TGV_U0 * n / (2.0 * math.pi * TGV_RE) + 0.5, 6),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["frac"], "traffic": traffic,
+                     "traffic_source": tsrc + " (rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel, copy-calibrated)"
+                     if traffic else None,
+                     "kernel": "fused collide-stream, row-tiled, one cell per thread (k_step_pencil<float, SMAG_FIXED, "
+                               "NONE>)",
+                     "kernel_ms": head["kernel_ms"], "kernel_ms_even_odd": head["kernel_ms_even_odd"],
+                     "bytes_per_launch": 152 * n ** 3},
         "wall_s": round(wall, 3),
-        "extras": extras,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.seed)
+        out["cpu_baseline"] = cpu_baseline_tgv()
+    if not args.no_extras:
+        ex = {}
+        lag = tgv_entry("f32", "lagged", args.steps, args.warmup)
+        ex["tgv256_f32_lagged"] = lag
+        for st in ("f16", "bf16"):
+            ex[f"tgv256_{st}"] = tgv_entry(st, "fixed", args.steps, args.warmup)
+            ex[f"tgv256_{st}_over_f32"] = round(ex[f"tgv256_{st}"]["mlups"] / head["mlups"], 3)
+        ex["tgv256_errors_after_1000_steps"] = tgv_errors(1000)
+        ex["error_note"] = ("relative L2 velocity error against the FP64-storage run (FP64 collision) of the same "
+                            "Smagorinsky form; FP64 storage is pinned to the FP64 oracle on a 32^3 vortex to 1e-12")
+        ex["c1_cavity64"] = cavity_c1(cpu=not args.no_cpu_baseline)
+        ex["c2_coupled_psc"] = coupled_c2(args.seed)
+        ex["c5_adjoint"] = adjoint_c5()
+        out["extras"] = ex
     print(json.dumps(out), flush=True)
 
 

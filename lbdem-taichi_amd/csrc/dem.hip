@@ -1301,7 +1301,7 @@ static int check_params(const lbdem_dem_params *P, const char *who) {
   LBDEM_REQUIRE(P->n >= 0, "%s: negative particle count", who);
   LBDEM_REQUIRE(P->T > 0 && (P->T & (P->T - 1)) == 0, "%s: hash table size %d is not a power of two", who, P->T);
   LBDEM_REQUIRE(P->cell > 0.0, "%s: cell size must be positive", who);
-  LBDEM_REQUIRE(P->K > 0, "%s: max_coordinate_number must be positive", who);
+  LBDEM_REQUIRE(P->K >= 0, "%s: max_coordinate_number must not be negative", who);
   LBDEM_REQUIRE(P->model == 0 || P->model == 1, "%s: model must be 0 (Hertz) or 1 (linear)", who);
   return LBDEM_OK;
 }

@@ -226,8 +226,10 @@ class DEMSolver:
         groups = np.zeros(n, dtype=np.int64) if groups is None else np.asarray(groups).reshape(n)
         self.n = n
         K = int(self.config.particle_props.max_coordinate_number)
-        if K <= 0:
-            raise ValueError("set_particle_properties(max_coordinate_number=...) must be positive")
+        if K < 0:
+            raise ValueError("set_particle_properties(max_coordinate_number=...) must not be negative")
+        # K = 0 (the reference's default, types.py:42) is accepted as there: no contact slots,
+        # so a particle-particle contact is counted as a slot overflow (dem.check() raises)
         self.K = K
         dev = self.device
         # density of the last particle is taken as the material density (demsolver.py:176-178, 228)

@@ -49,17 +49,25 @@ def test_cavity_f64_storage_matches_oracle_tightly(smag017):
     assert rel_err(gpu.vel.to_numpy(), orc.vel, fluid) < 1e-6
 
 
-@pytest.mark.parametrize("storage,tol", [("f16", 3e-2), ("bf16", 2e-1)])
+@pytest.mark.parametrize("storage,tol", [("f16", 1e-3), ("bf16", 1e-2)])
 def test_cavity_reduced_storage_close(smag017, storage, tol):
+    """Shifted 16-bit storage (FP32 collision) against the FP64 oracle, 16^3 cavity, 200 and
+    1,000 steps, relative max-norm and L2 velocity error.  FP16 holds the 1e-3 target (measured
+    2.9e-4); BF16's 8-bit significand does not (measured 3-4e-3), and is held to 1e-2."""
     from src.lbm3d.lbm_solver3d import BasicLattice3D
     orc = oracle_cavity(16)
     gpu = gpu_from_oracle(BasicLattice3D, orc, storage=storage)
     orc.initialize()
     gpu.initialize()
-    orc.step(200)
-    gpu.step(200)
     fluid = orc.CT == O.FLUID
-    assert rel_err(gpu.vel.to_numpy(), orc.vel, fluid) < tol
+    done = 0
+    for steps in (200, 1000):
+        orc.step(steps - done)
+        gpu.step(steps - done)
+        done = steps
+        u, r = gpu.vel.to_numpy()[fluid].astype(np.float64), orc.vel[fluid]
+        assert rel_err(u, r) < tol, steps
+        assert np.linalg.norm(u - r) / np.linalg.norm(r) < tol, steps
 
 
 def test_forced_poiseuille_matches_oracle_and_analytic():
