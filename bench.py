@@ -778,7 +778,7 @@ def main():
     head = tgv_entry("f32", "fixed", args.steps, args.warmup)
     wall = time.perf_counter() - t_wall
     gc.enable()
-    traffic, tsrc = None, os.path.join("profiles", "tgv256_collide_stream_traffic.json")
+    traffic, tsrc = None, os.path.join("profiles", "tgv256_f32_collide_stream_traffic.json")
     if os.path.exists(os.path.join(ROOT, tsrc)):
         with open(os.path.join(ROOT, tsrc)) as fh:
             traffic = json.load(fh).get("bytes_per_launch")

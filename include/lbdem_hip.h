@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 4
+#define LBDEM_ABI_VERSION 5
 
 /* return codes */
 #define LBDEM_OK 0
@@ -63,7 +63,8 @@ extern "C" {
                                   without it must be WALL or MOVING: its slots hold the
                                   populations bounced at it, not populations of its own */
 #define LBDEM_FLAG_NEAR 0x02   /* has a wall neighbour (set by lbdem_lbm_prepare_flags) */
-#define LBDEM_FLAG_WALL 0x04   /* bounce-back wall (CellType.OBSTACLE) */
+#define LBDEM_FLAG_WALL 0x04   /* bounce-back wall (CellType.OBSTACLE; CellType.FREE_SLIP, whose
+                                  reflection lbdem_lbm_free_slip applies after the step) */
 #define LBDEM_FLAG_MOVING 0x08 /* moving bounce-back wall (CellType.VEL_LADD), velocity in vel[] */
 #define LBDEM_FLAG_FLUID 0x10  /* rho,u are refreshed after streaming (CellType.FLUID) */
 #define LBDEM_FLAG_INLET 0x20  /* with MOVING: Ladd velocity inlet (CellType.VEL_INLET_LADD); its
@@ -163,6 +164,19 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
  * come from the neighbour's populations when it is FLUID, else from rho / vel. */
 int lbdem_lbm_boundaries(const lbdem_lattice *lat, const lbdem_step_args *a, const int32_t *cells,
                          const uint8_t *kinds, int32_t ncells, float *rho, float *vel, void *stream);
+
+/* Free-slip walls, after the step that produced the state after a->t steps and before
+ * lbdem_lbm_boundaries: the FREE_SLIP branch of BasicLattice3D.stream (src/lbm3d/lbm_solver3d.py:
+ * 230-236, mirror maps :50-57; also Unresolvedlattice3D.stream, src/unlbdem/vanslattice.py:
+ * 388-395).  Free-slip cells are WALL cells for the step (halfway bounce-back); this call then
+ * sets each listed post-stream population to the reflected post-collision one.  pairs: npairs
+ * records of four int32 -- target cell and direction (an active cell whose upstream neighbour is
+ * the free-slip cell), source cell and direction (a post-stream population of the same step
+ * holding the post-collision value fpc[x'][qsy[q]]; source cell -1: the value 0, the reference's
+ * fpc of a cell that never collides).  work: npairs doubles of scratch.  All sources are read
+ * before any target is written. */
+int lbdem_lbm_free_slip(const lbdem_lattice *lat, const lbdem_step_args *a, const int32_t *pairs, int32_t npairs,
+                        double *work, void *stream);
 
 /* rho, vel (AoS) of the post-stream state after t steps, at FLUID cells (all active cells
  * for VANS, with its force/volume-fraction correction).  If `mask` is non-NULL only cells

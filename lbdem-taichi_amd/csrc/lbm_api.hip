@@ -323,6 +323,25 @@ int lbdem_lbm_boundaries(const lbdem_lattice *lat, const lbdem_step_args *a, con
   return LBDEM_OK;
 }
 
+int lbdem_lbm_free_slip(const lbdem_lattice *lat, const lbdem_step_args *a, const int32_t *pairs, int32_t npairs,
+                        double *work, void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_free_slip");
+  if (rc) return rc;
+  LBDEM_REQUIRE(npairs >= 0, "lbdem_lbm_free_slip: negative pair count");
+  if (npairs == 0) return LBDEM_OK;
+  LBDEM_REQUIRE(pairs && work, "lbdem_lbm_free_slip: pairs and work are required");
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_free_slip: bad storage %d", lat->storage);
+  const unsigned blocks = (unsigned)((npairs + 255) / 256);
+  hipLaunchKernelGGL(t->free_slip_read, dim3(blocks), dim3(256), 0, as_stream(stream), p, pairs, (int)npairs, work);
+  LBDEM_CHECK_LAUNCH("lbdem_lbm_free_slip");
+  hipLaunchKernelGGL(t->free_slip_write, dim3(blocks), dim3(256), 0, as_stream(stream), p, pairs, (int)npairs,
+                     (const double *)work);
+  LBDEM_CHECK_LAUNCH("lbdem_lbm_free_slip");
+  return LBDEM_OK;
+}
+
 int lbdem_lbm_stress(const lbdem_lattice *lat, const lbdem_step_args *a, float *stress, void *stream) {
   KParams p;
   int rc = fill_params(lat, a, p, "lbdem_lbm_stress");

@@ -789,6 +789,56 @@ __global__ __launch_bounds__(256) void k_boundaries(const KParams p, const int32
   scatter_state<S>(p, F, c, fl, odd, f);
 }
 
+// ---------------------------------------------------------------------------------------
+// Free-slip walls after a step (specular reflection, the FREE_SLIP branch of
+// BasicLattice3D.stream, src/lbm3d/lbm_solver3d.py:230-236, mirror maps :50-57).  The fused step
+// treats a free-slip cell as a resting wall (halfway bounce-back); this pass then replaces each
+// population q of an active cell y whose upstream neighbour y - c_q is a free-slip cell by the
+// post-collision population qsy[q] of the cell x' = y - c_q with the wall-normal component of the
+// offset dropped (the reference pulls fpc[x'][qsy[q]]), or by 0 where x' does not collide (the
+// reference's never-written fpc).  Every such post-collision value is a post-stream population
+// of this step (the host finds which: the bounced one of x' where x' + c_qsy[q] is a wall, else
+// the one streamed into x' + c_qsy[q]), and sources can be targets of other pairs, so all
+// sources are read (k_free_slip_read) before any target is written (k_free_slip_write).
+// pairs: four int32 per pair -- target cell, target direction, source cell (-1: zero), source
+// direction.  Mirrored directions have the weight of the original, so the shifted stored values
+// copy unchanged.
+// ---------------------------------------------------------------------------------------
+// index into the population array of the post-stream population s of active cell z at the
+// state's parity (the address gather uses)
+__device__ __forceinline__ size_t post_stream_slot(const KParams &p, int z, int s, bool odd) {
+  if (s == 0) return (size_t)z;
+  const int o = kBcOpp[s];
+  const bool bounced = (p.flags[z] & LBDEM_FLAG_NEAR) && ((p.nmask[z] >> o) & 1u);
+  if (bounced == odd) return (size_t)s * p.ncell + z;  // bounced after an odd step, streamed after an even one
+  int i, j, k;
+  cell_ijk(p, z, i, j, k);
+  return (size_t)o * p.ncell + nbr_index(p, i, j, k, -kBcCx[s], -kBcCy[s], -kBcCz[s]);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_free_slip_read(const KParams p, const int32_t *pairs, int npairs,
+                                                        double *val) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= npairs) return;
+  const int q = pairs[4 * e + 1], z = pairs[4 * e + 2], s = pairs[4 * e + 3];
+  if (z < 0) {  // a population of 0, shifted by w_q rho0
+    const double w = q == 0 ? 1.0 / 3.0 : (q <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
+    val[e] = (double)(typename Store<S>::T)(-w * p.rho0);
+    return;
+  }
+  val[e] = (double)Store<S>::ld(reinterpret_cast<const S *>(p.f) + post_stream_slot(p, z, s, p.t_odd != 0));
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_free_slip_write(const KParams p, const int32_t *pairs, int npairs,
+                                                         const double *val) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= npairs) return;
+  const size_t a = post_stream_slot(p, pairs[4 * e], pairs[4 * e + 1], p.t_odd != 0);
+  Store<S>::st(reinterpret_cast<S *>(p.f) + a, (typename Store<S>::T)val[e]);
+}
+
 // Stress from the non-equilibrium populations, BasicLattice3D.compute_stress
 // (src/lbm3d/lbm_solver3d.py:347-405): sigma_ab = -(1 - omega/2) sum_q c_a c_b (f_q - feq_q), f
 // the populations after t steps, feq and omega those of the last collision.  With lagged

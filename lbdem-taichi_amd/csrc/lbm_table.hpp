@@ -6,6 +6,8 @@ namespace lbdem {
 
 using KernelFn = void (*)(const KParams);
 using BcKernelFn = void (*)(const KParams, const int32_t *, const uint8_t *, int, float *, float *);
+using FsReadFn = void (*)(const KParams, const int32_t *, int, double *);
+using FsWriteFn = void (*)(const KParams, const int32_t *, int, const double *);
 
 struct LbmKernels {
   KernelFn step[3][5][2];  // [collision][coupling][odd]; nullptr = unsupported combination
@@ -17,6 +19,8 @@ struct LbmKernels {
   KernelFn read_f;
   KernelFn write_f;
   BcKernelFn boundaries;
+  FsReadFn free_slip_read;
+  FsWriteFn free_slip_write;
   KernelFn stress;
   KernelFn reduce_partial;
 };
@@ -119,6 +123,8 @@ void add_hybrid_f64(LbmKernels &k);
       k.read_f = k_read_f<S>;                                                                   \
       k.write_f = k_write_f<S>;                                                                 \
       k.boundaries = k_boundaries<S>;                                                           \
+      k.free_slip_read = k_free_slip_read<S>;                                                   \
+      k.free_slip_write = k_free_slip_write<S>;                                                 \
       k.stress = k_stress<S>;                                                                   \
       k.reduce_partial = k_reduce_partial<S>;                                                   \
       return k;                                                                                 \

@@ -28,7 +28,7 @@ class Lattice(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class StepArgs(ctypes.Structure):
@@ -64,6 +64,7 @@ def lib():
         "lbdem_lbm_reduce": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, _vp]),
         "lbdem_lbm_prepare_flags": (ctypes.c_int, [P(Lattice), _vp, _vp, _vp]),
         "lbdem_lbm_boundaries": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, ctypes.c_int32, _vp, _vp, _vp]),
+        "lbdem_lbm_free_slip": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, ctypes.c_int32, _vp, _vp]),
         "lbdem_lbm_stress": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp]),
         "lbdem_lbm_plane_stress": (ctypes.c_int, [P(Lattice), _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     }

@@ -22,6 +22,50 @@ _TYPE_BITS = (CellType.FLUID | CellType.INTERFACE | CellType.GAS | CellType.OBST
               CellType.VEL_INLET_LADD)
 
 
+def free_slip_pairs(ct, active, solid, moving):
+    """Records (target cell, target direction, source cell, source direction), int32 (n, 4), of
+    the free-slip pass (lbdem_lbm_free_slip) for the FREE_SLIP branch of stream
+    (lbm_solver3d.py:230-236).  A target is population q of an active cell y whose upstream
+    neighbour m = y - c_q is a free-slip cell; the reference pulls fpc[x'][qsy[q]], x' being m
+    with its wall-normal index replaced by y's (the normal is the first of LEFT|RIGHT,
+    BOTTOM|TOP, BACK|FRONT set on m).  The source is where the step put that post-collision
+    value: the bounced population OPP[q'] of x' when x' + c_q' is a wall, else population q' of
+    x' + c_q'; -1 (the value 0, the reference's never-written fpc) when x' does not collide.
+    Indices wrap periodically like the reference's compute_neigh_index (:308-342)."""
+    shape = ct.shape
+    C = BasicLattice3D.c.astype(np.int64)
+    qsy = (BasicLattice3D.qsyx, BasicLattice3D.qsyy, BasicLattice3D.qsyz)
+    fs_cells = np.argwhere((ct & CellType.FREE_SLIP) != 0)
+    axis = np.full(len(fs_cells), -1, dtype=np.int64)
+    for a, bits in reversed(list(enumerate((CellType.LEFT | CellType.RIGHT, CellType.BOTTOM | CellType.TOP,
+                                            CellType.BACK | CellType.FRONT)))):
+        axis[(ct[tuple(fs_cells.T)] & bits) != 0] = a
+    out = []
+    for q in range(1, 19):
+        y = (fs_cells + C[q]) % shape
+        sel = active[tuple(y.T)]
+        if not sel.any():
+            continue
+        m, y, ax = fs_cells[sel], y[sel], axis[sel]
+        if (ax < 0).any():
+            raise NotImplementedError("a FREE_SLIP cell next to a fluid cell needs a face flag (LEFT, RIGHT, BOTTOM, "
+                                      "TOP, BACK or FRONT): without one the reference leaves the population as it was")
+        xp = m.copy()
+        xp[np.arange(len(m)), ax] = y[np.arange(len(m)), ax]
+        qp = np.array([qsy[a][q] for a in ax], dtype=np.int64)
+        coll = active[tuple(xp.T)]
+        z = (xp + C[qp]) % shape
+        bounced = solid[tuple(z.T)]
+        if (coll & bounced & moving[tuple(z.T)]).any():
+            raise NotImplementedError("a free-slip reflection whose source population bounced at a moving wall is not "
+                                      "supported")
+        lin = lambda v: (v[:, 0] * shape[1] + v[:, 1]) * shape[2] + v[:, 2]  # noqa: E731
+        src_cell = np.where(coll, np.where(bounced, lin(xp), lin(z)), -1)
+        src_dir = np.where(coll, np.where(bounced, BasicLattice3D.qinv[qp], qp), 0)
+        out.append(np.column_stack([lin(y), np.full(len(y), q), src_cell, src_dir]))
+    return np.concatenate(out).astype(np.int32) if out else np.zeros((0, 4), dtype=np.int32)
+
+
 class _PopulationField:
     """`lattice.f`: unshifted populations of the post-stream state, shape (Nx, Ny, Nz, 19)."""
 
@@ -165,17 +209,20 @@ class BasicLattice3D:
         kind = ct & _TYPE_BITS
         wet = {CellType.VEL_ZOUHE: _hip.BC_VEL_ZOUHE, CellType.Pre_ZOUHE: _hip.BC_PRE_ZOUHE,
                CellType.VEL_EXIT: _hip.BC_EXIT, CellType.VEL_INLET_LADD: _hip.BC_LADD_INLET}
-        supported = (kind == CellType.FLUID) | (kind == CellType.OBSTACLE) | (kind == CellType.VEL_LADD)
+        supported = ((kind == CellType.FLUID) | (kind == CellType.OBSTACLE) | (kind == CellType.VEL_LADD) |
+                     (kind == CellType.FREE_SLIP))
         for t in wet:
             supported |= kind == t
         if not supported.all():
             bad = np.unique(kind[~supported])
             raise NotImplementedError(
                 f"cell types {bad.tolist()} are not supported by the fused kernel (supported: FLUID, "
-                "OBSTACLE, VEL_LADD, VEL_ZOUHE, Pre_ZOUHE, VEL_EXIT, VEL_INLET_LADD; free-slip is not "
-                "implemented)")
+                "OBSTACLE, VEL_LADD, FREE_SLIP, VEL_ZOUHE, Pre_ZOUHE, VEL_EXIT, VEL_INLET_LADD, one each)")
         active = (ct & self._skip_types) == 0
-        wall = kind == CellType.OBSTACLE
+        free_slip = kind == CellType.FREE_SLIP
+        if (free_slip & active).any():
+            raise NotImplementedError("FREE_SLIP cells must be among the skipped cell types")
+        wall = (kind == CellType.OBSTACLE) | free_slip
         inlet = kind == CellType.VEL_INLET_LADD
         if (inlet & active).any():
             raise NotImplementedError("VEL_INLET_LADD cells must be among the skipped cell types")
@@ -184,6 +231,11 @@ class BasicLattice3D:
         flags = (active * _hip.FLAG_ACTIVE | wall * _hip.FLAG_WALL | moving * _hip.FLAG_MOVING |
                  inlet * _hip.FLAG_INLET | fluid * _hip.FLAG_FLUID).astype(np.uint8)
         self._flags.copy_(torch.from_numpy(flags.reshape(-1)))
+        pairs = free_slip_pairs(ct, active, wall | moving, moving) if free_slip.any() else None
+        self._fs_pairs = None if pairs is None or not len(pairs) else \
+            torch.from_numpy(pairs.reshape(-1)).to(self.device)
+        self._fs_work = None if self._fs_pairs is None else \
+            torch.empty(len(pairs), dtype=torch.float64, device=self.device)
         # wet-node boundary cells, applied after every step: type | face << 4, the face being the
         # first of LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT set (7: none)
         bc_type = np.zeros(kind.shape, dtype=np.int64)
@@ -271,6 +323,10 @@ class BasicLattice3D:
             e1.record()
             ev.append((e0, e1))
         self.t += 1
+        if self._fs_pairs is not None:  # free-slip reflection on the post-stream state (:230-236)
+            _hip.check(_hip.lib().lbdem_lbm_free_slip(self._lat_c, self._args_c(), _hip.ptr(self._fs_pairs),
+                                                      int(self._fs_work.numel()), _hip.ptr(self._fs_work),
+                                                      _hip.stream()), "lbdem_lbm_free_slip")
         nbc = int(self._bc_cells.numel())
         if nbc:  # wet-node boundaries on the post-stream state (lbm_solver3d.py:241-250)
             _hip.check(_hip.lib().lbdem_lbm_boundaries(self._lat_c, self._args_c(), _hip.ptr(self._bc_cells),

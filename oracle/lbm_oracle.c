@@ -175,6 +175,19 @@ void SFX(lbo_collide_weighted)(int nx, int ny, int nz, const int32_t *CT, int32_
   }
 }
 
+#define CT_RIGHT 4096
+#define CT_LEFT 8192
+#define CT_TOP 16384
+#define CT_BOTTOM 32768
+#define CT_FRONT 65536
+#define CT_BACK 131072
+
+/* mirror maps qsyx / qsyy / qsyz, src/lbm3d/lbm_solver3d.py:50-57: the direction with the x / y /
+ * z component of c reversed */
+static const int QSY[3][Q] = {{0, 2, 1, 3, 4, 5, 6, 10, 9, 8, 7, 14, 13, 12, 11, 15, 16, 17, 18},
+                              {0, 1, 2, 4, 3, 5, 6, 9, 10, 7, 8, 11, 12, 13, 14, 18, 17, 16, 15},
+                              {0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16}};
+
 /* compute_neigh_index, src/lbm3d/lbm_solver3d.py:308-342 */
 static void neigh(int nx, int ny, int nz, int q, int i, int j, int k, int *a, int *b, int *d) {
   int in = i - CX[q], jn = j - CY[q], kn = k - CZ[q];
@@ -211,6 +224,12 @@ void SFX(lbo_stream)(int nx, int ny, int nz, const int32_t *CT, int32_t skip, co
           } else if (t & CT_VEL_INLET_LADD) {  /* the inlet cell's density (:225-229) */
             const REAL cu = CX[q] * vel[3 * m] + CY[q] * vel[3 * m + 1] + CZ[q] * vel[3 * m + 2];
             f[Q * c + q] = fpc[Q * c + QINV[q]] + 2.0 * W[q] * rho[m] * cu / (1.0 / 3.0);
+          } else if (t & CT_FREE_SLIP) {  /* specular reflection (:230-236) */
+            /* the neighbour index with the wall-normal component kept at this cell's: a, b, d
+             * with one of them replaced by i, j or k; fpc of a cell that never collides is 0 */
+            if (t & (CT_LEFT | CT_RIGHT)) f[Q * c + q] = fpc[Q * IDX(i, b, d) + QSY[0][q]];
+            else if (t & (CT_BOTTOM | CT_TOP)) f[Q * c + q] = fpc[Q * IDX(a, j, d) + QSY[1][q]];
+            else if (t & (CT_BACK | CT_FRONT)) f[Q * c + q] = fpc[Q * IDX(a, b, k) + QSY[2][q]];
           }
           /* other neighbour types leave f[q] unchanged (lbm_solver3d.py:216-236) */
         }
@@ -245,12 +264,6 @@ void SFX(lbo_stream)(int nx, int ny, int nz, const int32_t *CT, int32_t skip, co
  * LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT (the reference's if-chain).  Zou-He for the face with
  * normal axis a: the populations with c_a = -s (s = +1 on the low face) are the known incoming
  * ones, those with c_a = s are rebuilt from their opposites. */
-#define CT_RIGHT 4096
-#define CT_LEFT 8192
-#define CT_TOP 16384
-#define CT_BOTTOM 32768
-#define CT_FRONT 65536
-#define CT_BACK 131072
 
 /* face index 0..5 = LEFT, RIGHT, BOTTOM, TOP, BACK, FRONT; -1 none */
 static int bc_face(int32_t ct) {

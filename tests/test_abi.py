@@ -45,7 +45,8 @@ def test_python_binding_declares_every_header_function():
     known = set(H._EXTRA_SIGNATURES) | {
         "lbdem_abi_version", "lbdem_last_error", "lbdem_lbm_init", "lbdem_lbm_step", "lbdem_lbm_macros",
         "lbdem_lbm_read_f", "lbdem_lbm_write_f", "lbdem_reduce_work_doubles", "lbdem_lbm_reduce",
-        "lbdem_lbm_prepare_flags", "lbdem_lbm_boundaries", "lbdem_lbm_stress", "lbdem_lbm_plane_stress"}
+        "lbdem_lbm_prepare_flags", "lbdem_lbm_boundaries", "lbdem_lbm_free_slip", "lbdem_lbm_stress",
+        "lbdem_lbm_plane_stress"}
     missing = [n for n in declared_functions() if n not in known]
     assert not missing, f"no ctypes prototype for {missing}"
 
