@@ -222,6 +222,15 @@ int64_t lbdem_reduce_work_doubles(const lbdem_lattice *lat);
 int lbdem_lbm_reduce(const lbdem_lattice *lat, const lbdem_step_args *a, double *work,
                      double *out, void *stream);
 
+/* lbdem_lbm_reduce without the caller's work buffer, and without any allocation: the
+ * per-workgroup partials live in a file-scope device array of the library (5 MiB, one per GPU,
+ * kept until the process ends), for lattices of at most 2^25 cells (larger ones are rejected
+ * with LBDEM_E_ARG).  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Calls on one GPU share that GPU's array: calls that overlap on one GPU (other
+ * streams, other host threads) race; calls in stream order do not. */
+int lbdem_lbm_reduce_static(const lbdem_lattice *lat, const lbdem_step_args *a, double *out, void *stream);
+
 /* ------------------------------------------------------------------------------------ */
 /* grain / particle state shared by the couplings and the DEM (device pointers, FP64)     */
 /* ------------------------------------------------------------------------------------ */

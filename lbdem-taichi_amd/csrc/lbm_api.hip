@@ -1,6 +1,7 @@
 // extern "C" entry points of the lattice core (see include/lbdem_hip.h).
 #include <stdarg.h>
 #include <stdio.h>
+#include <mutex>
 #include <string>
 #include "common.hpp"
 #include "lbm_table.hpp"
@@ -16,6 +17,39 @@ void set_error(const char *fmt, ...) {
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
   g_last_error = buf;
+}
+
+// Scratch of lbdem_lbm_reduce_static: the per-workgroup partials of the reduction (five doubles
+// per 256 cells), for lattices of at most kReduceStaticMaxCells cells.  One copy per GPU, held
+// until the process ends.  Host code never uses the name `scratch` as an address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress, taken once per device.
+constexpr int64_t kReduceStaticMaxCells = int64_t(1) << 25;
+constexpr int64_t kReduceStaticDoubles = (kReduceStaticMaxCells / 256) * 5;
+static __device__ __align__(256) double scratch[kReduceStaticDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices];
+
+// device address of the current device's scratch (nullptr, with the error set, on failure)
+static double *scratch_for_current_device(const char *who) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("%s: no usable current device (%s)", who, e != hipSuccess ? hipGetErrorString(e) : "index");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("%s: hipGetSymbolAddress(scratch) failed: %s", who, hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  return g_scratch_ptr[dev];
 }
 
 static const LbmKernels *table_for(int storage) {
@@ -388,6 +422,29 @@ int lbdem_lbm_reduce(const lbdem_lattice *lat, const lbdem_step_args *a, double 
   const int nblocks = (p.ncell + 255) / 256;
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, as_stream(stream), work, nblocks, out);
   LBDEM_CHECK_LAUNCH("lbdem_lbm_reduce");
+  return LBDEM_OK;
+}
+
+int lbdem_lbm_reduce_static(const lbdem_lattice *lat, const lbdem_step_args *a, double *out, void *stream) {
+  KParams p;
+  int rc = fill_params(lat, a, p, "lbdem_lbm_reduce_static");
+  if (rc) return rc;
+  LBDEM_REQUIRE(out, "lbdem_lbm_reduce_static: out is required");
+  LBDEM_REQUIRE((int64_t)p.ncell <= kReduceStaticMaxCells,
+                "lbdem_lbm_reduce_static: %d cells exceed the static scratch (%lld cells; use lbdem_lbm_reduce)",
+                p.ncell, (long long)kReduceStaticMaxCells);
+  const LbmKernels *t = table_for(lat->storage);
+  LBDEM_REQUIRE(t, "lbdem_lbm_reduce_static: bad storage %d", lat->storage);
+  double *work = scratch_for_current_device("lbdem_lbm_reduce_static");
+  if (!work) return LBDEM_E_HIP;
+  // every workgroup writes all five of its partials before the final pass reads them: nothing
+  // in the scratch needs clearing
+  p.work = work;
+  rc = launch(t->reduce_partial, p.ncell, p, stream, "lbdem_lbm_reduce_static");
+  if (rc) return rc;
+  const int nblocks = (p.ncell + 255) / 256;
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, as_stream(stream), work, nblocks, out);
+  LBDEM_CHECK_LAUNCH("lbdem_lbm_reduce_static");
   return LBDEM_OK;
 }
 

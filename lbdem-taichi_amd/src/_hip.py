@@ -62,6 +62,7 @@ def lib():
         "lbdem_lbm_write_f": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, ctypes.c_int32, _vp]),
         "lbdem_reduce_work_doubles": (ctypes.c_int64, [P(Lattice)]),
         "lbdem_lbm_reduce": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, _vp]),
+        "lbdem_lbm_reduce_static": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp]),
         "lbdem_lbm_prepare_flags": (ctypes.c_int, [P(Lattice), _vp, _vp, _vp]),
         "lbdem_lbm_boundaries": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, _vp, ctypes.c_int32, _vp, _vp, _vp]),
         "lbdem_lbm_free_slip": (ctypes.c_int, [P(Lattice), P(StepArgs), _vp, ctypes.c_int32, _vp, _vp]),

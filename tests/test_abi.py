@@ -24,6 +24,10 @@ def test_header_declares_entry_points():
 
 def test_library_exports_every_declared_symbol():
     assert os.path.exists(LIB), "build liblbdem_hip.so first (python __graft_entry__.py)"
+    # torch first: its HIP runtime (same soname) then serves the library too; a library loaded
+    # before torch brings a second HIP runtime into the process, and GPU calls made later in
+    # the same process see no device
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB)
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, f"not exported: {missing}"
@@ -45,6 +49,7 @@ def test_python_binding_declares_every_header_function():
     known = set(H._EXTRA_SIGNATURES) | {
         "lbdem_abi_version", "lbdem_last_error", "lbdem_lbm_init", "lbdem_lbm_step", "lbdem_lbm_macros",
         "lbdem_lbm_read_f", "lbdem_lbm_write_f", "lbdem_reduce_work_doubles", "lbdem_lbm_reduce",
+        "lbdem_lbm_reduce_static",
         "lbdem_lbm_prepare_flags", "lbdem_lbm_boundaries", "lbdem_lbm_free_slip", "lbdem_lbm_stress",
         "lbdem_lbm_plane_stress"}
     missing = [n for n in declared_functions() if n not in known]
@@ -96,3 +101,19 @@ def test_integration_stub_structs_match_the_binding():
         want = [(f, C.sizeof(t)) for f, t in getattr(H, name)._fields_]
         assert got == want, name
         assert C.sizeof(ns[name]) == C.sizeof(getattr(H, name))
+
+
+def test_reduce_static_rejects_a_lattice_beyond_its_scratch():
+    """The allocation-free reduction checks the lattice size on the host, before any HIP call."""
+    import src._hip as H
+    L = H.lib()
+    lat = H.Lattice()
+    lat.nx, lat.ny, lat.nz = 512, 256, 257   # 2^25 + 2^17 cells
+    lat.omega0 = 1.0
+    args = H.StepArgs()
+    dummy = ctypes.c_double(0.0)
+    args.f = args.flags = args.nmask = ctypes.addressof(dummy)
+    out = (ctypes.c_double * 5)()
+    rc = L.lbdem_lbm_reduce_static(ctypes.byref(lat), ctypes.byref(args), ctypes.cast(out, ctypes.c_void_p), None)
+    assert rc == -1
+    assert b"exceed the static scratch" in L.lbdem_last_error()
