@@ -506,7 +506,8 @@ def hybrid_study(steps=10, warmup=2, subcycles=200, cpu=True):
 def tgv_variants_study(n=256):
     """Collide-stream GLUPS of each storage type and row-tiled variant on the 256^3 TGV."""
     cases = [("f32", "fixed", 2), ("f32", "fixed", 3), ("f32", "lagged", 2), ("bf16", "fixed", 4),
-             ("bf16", "fixed", 3), ("f16", "fixed", 4), ("f16", "fixed", 3), ("f64", "fixed", 3)]
+             ("bf16", "fixed", 3), ("f16", "fixed", 4), ("f16", "fixed", 3), ("bf16", "lagged", 4),
+             ("f16", "lagged", 4), ("f64", "fixed", 3), ("f64", "lagged", 3)]
     out = []
     for storage, smag, kernel in cases:
         r = tgv_entry(storage, smag, 20, 4, n=n, kernel=kernel)
@@ -572,37 +573,53 @@ def precision_study(n=256, steps=20000, every=500, checkpoints=(1000, 5000, 1000
             "smagorinsky": {"Cs": 0.1, "mode": smag}, "results": out}
 
 
-def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=10):
+def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=10, relax=4000, K=64):
     """Unresolved EqIBM LBM-DEM at its full single-GPU shape: 128^3 lattice (walls on all faces),
-    200,000 spheres with log-normal diameters (mean 0.5 lu, sigma_log 0.2, clipped below 1 lu)
-    seeded uniformly in a 64 x 64 x 9 lu bed on the floor (solid fraction ~0.4, overlaps allowed),
-    density ratio 2.5, gravity -1e-5 lu, soft linear contacts (k_n = k_t = 1e-3, damping ratio 0.3,
-    friction 0.5: the random bed's overlaps relax without blowing the bed apart over the timed
-    steps), broad phase with cells one maximum diameter wide and the 27-cell probe, 10 DEM
-    substeps per lattice step, FP32 populations.  A step is: DEM substeps (general path),
-    collide-stream with the Tenneti weights, update_coupling (spreading, finalize, drag).
-    Reports per-phase times (HIP events) and the step's MLUPS."""
+    200,000 spheres with log-normal diameters (mean 0.5 lu, sigma_log 0.2, clipped below 1 lu) in
+    a 64 x 64 x 9 lu bed on the floor (solid fraction 0.4, porosity 0.6), density ratio 2.5, soft
+    linear contacts (k_n = k_t = 1e-3, damping ratio 0.3, friction 0.5), broad phase with cells
+    one maximum diameter wide and the 27-cell probe, 10 DEM substeps per lattice step, FP32
+    populations.  The seeded positions overlap; `relax` DEM substeps without gravity (not timed)
+    push the overlaps apart first, and the status counters are cleared after them, so the timed
+    steps start from a bed without large overlaps and must lose no contact (dem.check()).  A
+    timed step is: DEM substeps (general path, gravity -1e-5 lu), collide-stream with the
+    Tenneti weights, update_coupling (spreading, finalize, drag).  Reports per-phase times (HIP
+    events), the step's MLUPS and the DEM status counters of the timed steps."""
     import numpy as np
     import torch
-    from src.dem3d.demconfig import DEMSolverConfig, DomainBounds, LinearContactConfig
+    from src.dem3d.demconfig import DEMSolverConfig, DomainBounds, LinearContactConfig, Vector3
     from src.dem3d.demsolver import DEMSolver
     from src.unlbdem.eqlattice import EqIMBlattice3D
     rng = np.random.default_rng(seed)
     d = np.minimum(np.exp(np.log(0.5) - 0.02 + 0.2 * rng.standard_normal(n)), 0.999)
     rad = 0.5 * d
     L = np.array([N - 2.0, N - 2.0, N - 2.0])
-    lo = np.array([(L[0] - 64) / 2, (L[1] - 64) / 2, 0.0]) + 0.5
-    pos = lo + rng.random((n, 3)) * np.array([64.0, 64.0, 9.0])
+    lo = np.array([(L[0] - 64) / 2, (L[1] - 64) / 2, 0.0])
+    pos = lo + rad[:, None] + rng.random((n, 3)) * (np.array([64.0, 64.0, 9.0]) - 2 * rad[:, None])
     mass = 2.5 * 4.0 / 3.0 * np.pi * rad ** 3
     cm = LinearContactConfig(stiffness_normal=1e-3, stiffness_tangential=1e-3, damping_normal=0.3,
                              damping_tangential=0.3, pp_friction=0.5, pw_friction=0.5)
     cfg = DEMSolverConfig(domain=DomainBounds(0.0, L[0], 0.0, L[1], 0.0, L[2]), dt=1.0 / subcycles,
-                          gravity=(0.0, 0.0, -1e-5), contact_model=cm)
-    cfg.set_particle_properties(elastic_modulus=1.0, poisson_ratio=0.3, max_coordinate_number=32)
+                          gravity=(0.0, 0.0, 0.0), contact_model=cm)
+    cfg.set_particle_properties(elastic_modulus=1.0, poisson_ratio=0.3, max_coordinate_number=K)
     cfg.set_wall_properties(elastic_modulus=1.0, poisson_ratio=0.3, density=2.5)
     dem = DEMSolver(cfg)
     dem.init_particle_arrays(radius=rad, mass=mass, position=pos, cell_size=float(d.max()), probe27=True)
     dem.set_contact_model("linear")
+    t0 = time.perf_counter()
+    relax_status = []
+    for r in range(0, relax, 500):
+        dem.run_substeps(min(500, relax - r))
+        torch.cuda.synchronize()
+        relax_status.append(dem.status()["pp_overlap_warnings"])
+        dem.reset_status()
+        print(f"unresolved study: relaxation {r + 500} substeps, overlap warnings in them {relax_status[-1]}",
+              file=sys.stderr, flush=True)
+    relax_s = time.perf_counter() - t0
+    dem.gf.velocity.fill(0.0)
+    dem.gf.omega.fill(0.0)
+    cfg.gravity = Vector3(0.0, 0.0, -1e-5)
+    dem.invalidate()
     lat = EqIMBlattice3D(N, N, N, 1.0 / 0.6, 1.0, 1.0, 1.0, dem)
     lat.CT.from_numpy(walls_ct(N, N, N))
     lat.initialize_complete()
@@ -630,6 +647,7 @@ def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=
     for _ in range(warmup):
         step(False)
     torch.cuda.synchronize()
+    dem.reset_status()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(steps):
@@ -638,12 +656,14 @@ def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     per = {k: round(sum(a.elapsed_time(b) for a, b in v) / len(v), 4) for k, v in ev.items()}
-    st = dem.status()
+    st = dem.check()
     return {"study": "unresolved", "lattice": N, "particles": n, "steps": steps, "subcycles": subcycles,
+            "bed": "64 x 64 x 9 lu, solid fraction 0.40", "relaxation_substeps": relax,
+            "relaxation_s": round(relax_s, 2), "relaxation_overlap_warnings_per_500": relax_status,
             "ms_per_step": round(ms, 4), "mlups": round(N ** 3 / ms / 1e3, 1), "phase_ms": per,
             "dem_substep_ms": round(per["dem"] / subcycles, 4),
             "dem_particle_substeps_per_s": round(n * subcycles / (per["dem"] * 1e-3), 1),
-            "dem_status": st, "eqibm_clamped_cells": lat.clamp_count()}
+            "dem_status_timed_steps": st, "eqibm_clamped_cells": lat.clamp_count()}
 
 
 def adjoint_study(n=96, steps=2000, K=100, nsph=32, R=6.0, seed=7, n_fd_cells=16):
@@ -699,36 +719,48 @@ def adjoint_study(n=96, steps=2000, K=100, nsph=32, R=6.0, seed=7, n_fd_cells=16
             u = ch.forward((Fx, 0.0, 0.0), tv, torch.tensor(ev, dtype=torch.float64, device="cuda"), steps)
             return float(((u - tg) ** 2).mean())
 
+    # central differences with steps h and h/2 and their Richardson extrapolation
+    # (4 D(h/2) - D(h)) / 3, which removes the O(h^2) truncation term: at partly covered cells
+    # the loss is strongly curved in eps, and a plain central difference with h = 1e-3 is off by
+    # ~1e-3 there (its error falls fourfold per halving of h)
+    def central(fn, x0, h):
+        d1 = (fn(x0 + h) - fn(x0 - h)) / (2 * h)
+        d2 = (fn(x0 + h / 2) - fn(x0 - h / 2)) / h
+        return d1, d2, (4 * d2 - d1) / 3
+
     fd = []
-    h = 1e-4 * 1e-6
-    fd.append(("F_x", g64[0][0], (L(1e-6 + h, 0.6, eps0) - L(1e-6 - h, 0.6, eps0)) / (2 * h)))
-    h = 1e-4 * 0.6
-    fd.append(("tau", g64[1], (L(1e-6, 0.6 + h, eps0) - L(1e-6, 0.6 - h, eps0)) / (2 * h)))
+    fd.append(("F_x", g64[0][0], *central(lambda v: L(v, 0.6, eps0), 1e-6, 1e-4 * 1e-6)))
+    fd.append(("tau", g64[1], *central(lambda v: L(1e-6, v, eps0), 0.6, 1e-4 * 0.6)))
     fluid = np.argwhere((eps0 > 0) & (eps0 < 1))
     pick = fluid[rng.choice(len(fluid), n_fd_cells // 2, replace=False)]
     free = np.argwhere(eps0 == 0)
     free = free[(free[:, 1] > 0) & (free[:, 1] < n - 1) & (free[:, 0] > n - 20)]
     pick = list(pick) + list(free[rng.choice(len(free), n_fd_cells - len(pick), replace=False)])
-    # eps steps of 1e-3: the loss changes by ~1e-14 relative for the smallest gradients checked,
-    # which a 1e-4 step would leave at the rounding level of the 2,000-step FP64 forward run
+    # eps steps of 1e-3 and 5e-4: the loss changes by ~1e-14 relative for the smallest gradients
+    # checked, which much smaller steps would leave at the rounding level of the FP64 forward run
+
+    def with_eps(c, v):
+        e = eps0.copy()
+        e[c] = v
+        return L(1e-6, 0.6, e)
+
     for c in pick:
         c = tuple(int(v) for v in c)
-        ep, em = eps0.copy(), eps0.copy()
-        ep[c] += 1e-3
-        em[c] -= 1e-3
-        fd.append((f"eps{c}", g64[2][c], (L(1e-6, 0.6, ep) - L(1e-6, 0.6, em)) / 2e-3))
-    rel = [abs(a - b) / max(abs(b), 1e-30) for _, a, b in fd]
-    gscale = max(abs(b) for _, _, b in fd)
-    absrel = [abs(a - b) / gscale for _, a, b in fd]
+        fd.append((f"eps{c}", g64[2][c], *central(lambda v, c=c: with_eps(c, v), float(eps0[c]), 1e-3)))
+    rel_h = [abs(a - d1) / max(abs(d1), 1e-30) for _, a, d1, _, _ in fd]
+    rel = [abs(a - r) / max(abs(r), 1e-30) for _, a, _, _, r in fd]
+    gscale = max(abs(r) for *_, r in fd)
+    absrel = [abs(a - r) / gscale for _, a, _, _, r in fd]
     return {"study": "adjoint", "lattice": n, "steps": steps, "checkpoint_every": K, "spheres": nsph,
             "loss": loss64,
             "fp32": {"forward_ms_per_step": round(fwd32 * 1e3, 4), "adjoint_ms_per_step": round(adj32 * 1e3, 4),
                      "adjoint_over_forward": round(adj32 / fwd32, 3)},
             "fp64": {"forward_ms_per_step": round(fwd64 * 1e3, 4), "adjoint_ms_per_step": round(adj64 * 1e3, 4),
                      "adjoint_over_forward": round(adj64 / fwd64, 3)},
-            "fd_check_fp64": {"max_rel_error": max(rel), "max_error_over_largest_gradient": max(absrel),
-                              "n_checked": len(fd),
-                              "entries": [(k, a, b) for k, a, b in fd]},
+            "fd_check_fp64": {"max_rel_error_richardson": max(rel), "max_rel_error_central_h": max(rel_h),
+                              "max_error_over_largest_gradient": max(absrel), "n_checked": len(fd),
+                              "entries": [{"entry": k, "adjoint": a, "central_h": d1, "central_h2": d2,
+                                           "richardson": r} for k, a, d1, d2, r in fd]},
             "fp32_vs_fp64": {"F": float(np.abs(g32[0] - g64[0]).max() / np.abs(g64[0]).max()),
                              "tau": abs(g32[1] - g64[1]) / abs(g64[1]),
                              "eps": float(np.abs(g32[2] - g64[2]).max() / np.abs(g64[2]).max())}}
@@ -745,6 +777,8 @@ def main():
     ap.add_argument("--study", choices=["precision", "adjoint", "unresolved", "tgv", "channel", "hybrid", "coupled"],
                     default=None,
                     help="run a study instead of the benchmark and print its JSON summary")
+    ap.add_argument("--relax", type=int, default=4000,
+                    help="unresolved study: DEM substeps that relax the seeded bed before timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 512 MiB three times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -755,12 +789,10 @@ def main():
 
     import torch
     torch.cuda.set_device(0)
-    studies = {"precision": precision_study, "adjoint": adjoint_study, "unresolved": unresolved_study,
+    studies = {"precision": precision_study, "adjoint": adjoint_study,
+               "unresolved": lambda: unresolved_study(relax=args.relax),
                "tgv": tgv_variants_study, "channel": channel_study, "hybrid": hybrid_study,
                "coupled": lambda: coupled_c2(args.seed, args.steps, args.warmup)}
-    if args.study:
-        print(json.dumps(studies[args.study]()), flush=True)
-        return
     if args.calibrate_copy:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -768,6 +800,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.study:
+        print(json.dumps(studies[args.study]()), flush=True)
+        return
 
     n = TGV_N
     # Python's cyclic garbage collector is off in the timed regions (as in timeit): a full

@@ -45,6 +45,11 @@ void set_error(const char *fmt, ...);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Largest workgroup of the row-tiled collide-stream with V cells per thread.  The two- and
+// four-cell variants are capped at 256 threads: under a 1024-thread bound the compiler limits
+// them to 64 VGPRs, which the lagged Smagorinsky collision (and FP64) overflows into scratch.
+constexpr int pencil_max_threads(int V) { return V == 1 ? 1024 : 256; }
+
 // ---------------------------------------------------------------------------------------
 // storage types: load/store with conversion to the compute type
 // ---------------------------------------------------------------------------------------

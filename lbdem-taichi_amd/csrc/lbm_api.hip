@@ -176,25 +176,33 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *work, int nb
   if (threadIdx.x < 5) out[threadIdx.x] = sh[0][threadIdx.x];
 }
 
-// NEAR bit and wall-neighbour mask of every cell (bit q: x + c_q is a wall; kNbrMoving: a
-// moving one).  Each thread rewrites only its own flag byte and reads only the neighbours'
-// WALL / MOVING bits, which this kernel does not change.
+// NEAR bit and neighbour mask of every cell.  Active cells: bit q set where x + c_q is a wall
+// (kNbrMoving: some neighbour moves), NEAR where any is.  Wall and moving-wall cells: bit q set
+// where x + c_q is active -- the targets whose slots a wall must push back unchanged in the odd
+// row-tiled step (the other targets are walls, whose slots hold nothing that is read); no NEAR.
+// Each thread rewrites only its own flag byte and reads only the neighbours' ACTIVE / WALL /
+// MOVING bits, which this kernel does not change.
 __global__ __launch_bounds__(256) void k_prepare_flags(const KParams p, uint8_t *flags, uint32_t *nmask) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= p.ncell) return;
   const uint8_t fl = flags[c];
   uint32_t m = 0u;
-  if (fl & LBDEM_FLAG_ACTIVE) {
+  const bool act = fl & LBDEM_FLAG_ACTIVE;
+  if (act || (fl & (LBDEM_FLAG_WALL | LBDEM_FLAG_MOVING))) {
     int i, j, k;
     cell_ijk(p, c, i, j, k);
 #pragma unroll
     for (int q = 1; q < Q; ++q) {
       const uint8_t fn = flags[nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q])];
-      if (fn & (LBDEM_FLAG_WALL | LBDEM_FLAG_MOVING)) m |= 1u << q;
-      if (fn & LBDEM_FLAG_MOVING) m |= kNbrMoving;
+      if (act) {
+        if (fn & (LBDEM_FLAG_WALL | LBDEM_FLAG_MOVING)) m |= 1u << q;
+        if (fn & LBDEM_FLAG_MOVING) m |= kNbrMoving;
+      } else if (fn & LBDEM_FLAG_ACTIVE) {
+        m |= 1u << q;
+      }
     }
   }
-  const uint8_t near = m ? LBDEM_FLAG_NEAR : 0;
+  const uint8_t near = (act && m) ? LBDEM_FLAG_NEAR : 0;
   if ((fl & LBDEM_FLAG_NEAR) != near) flags[c] = (uint8_t)((fl & ~LBDEM_FLAG_NEAR) | near);
   nmask[c] = m;
 }
@@ -273,15 +281,23 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
     else if (lat->storage == LBDEM_STORE_F64) kern = 3;
     else kern = lat->nz % 4 == 0 ? 4 : 3;
   }
-  const int V = kern == 4 ? 4 : kern == 3 ? 2 : 1;
   const int tsize = lat->storage == LBDEM_STORE_F64 ? 8 : 4;
+  auto fits = [&](int v) {
+    const int b = ((lat->nz / v + 63) / 64) * 64;
+    return lat->nz % v == 0 && b <= pencil_max_threads(v) && (size_t)10 * b * v * tsize <= 65536;
+  };
+  if (lat->kernel == 0) {  // automatic: the preferred width, else a narrower one that fits the row
+    while (kern > 2 && !fits(kern == 4 ? 4 : 2)) --kern;
+  }
+  const int V = kern == 4 ? 4 : kern == 3 ? 2 : 1;
   const int block = ((lat->nz / V + 63) / 64) * 64;
   const size_t lds = p.t_odd ? (size_t)10 * block * V * tsize : 0;
-  const bool pencil_ok = (lat->nz % V == 0) && block <= 1024 && lds <= 65536;
+  const bool pencil_ok = fits(V);
   KernelFn kp = V == 4 ? t->pencil4[coll][lat->coupling][p.t_odd]
                 : V == 2 ? t->pencil2[coll][lat->coupling][p.t_odd] : t->pencil[coll][lat->coupling][p.t_odd];
   LBDEM_REQUIRE(lat->kernel < 2 || (pencil_ok && kp), "lbdem_lbm_step: row-tiled kernel %d needs nz %% %d == 0, a "
-                "row of at most %d cells and (4 cells per thread) 32-bit or 16-bit storage", kern, V, 1024 * V);
+                "row of at most %d cells and (4 cells per thread) 32-bit or 16-bit storage", kern, V,
+                pencil_max_threads(V) * V);
   if (kern >= 2 && pencil_ok && kp) {
     void *args[] = {&p};
     hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(kp), dim3((unsigned)(lat->nx * lat->ny)),

@@ -1009,7 +1009,7 @@ struct Vec {
 };
 
 template <typename S, int COLL, int COUP, bool ODD, int V>
-__global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
+__global__ __launch_bounds__(pencil_max_threads(V)) void k_step_pencil(const KParams p) {
   using T = typename Store<S>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   T *buf = reinterpret_cast<T *>(smem);  // [kNZ][ncz]
@@ -1032,10 +1032,14 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
     act[v] = fl[v] & LBDEM_FLAG_ACTIVE;
   }
   uint32_t nmv[V];  // wall-neighbour bits (NEAR cells only)
+  uint32_t tmv[V];  // wall cells: the active neighbours (odd step: slots to push back unchanged)
   uint32_t nm_any = 0u;  // union over this thread's cells: the wall paths below branch on it
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    nmv[v] = (fl[v] & LBDEM_FLAG_NEAR) ? p.nmask[base + k0 + v] : 0u;
+    const uint32_t m = (fl[v] & (LBDEM_FLAG_NEAR | LBDEM_FLAG_WALL | LBDEM_FLAG_MOVING)) ? p.nmask[base + k0 + v]
+                                                                                          : 0u;
+    nmv[v] = (fl[v] & LBDEM_FLAG_NEAR) ? m : 0u;
+    tmv[v] = act[v] ? 0u : m;
     nm_any |= nmv[v];
   }
   T f[V][Q];
@@ -1104,11 +1108,14 @@ __global__ __launch_bounds__(1024) void k_step_pencil(const KParams p) {
             if ((nm >> kOPP[q]) & 1u) f[v][q] = Store<S>::ld(F + q * N + base + k);
           });
         } else if (!act[v]) {
-          // A wall pushes back what its targets hold: a target x + c_q that is a fluid cell
+          // A wall pushes back what its targets hold: a target x + c_q that is an active cell
           // reads that slot (its bounced population) in this step, and no other thread writes it.
-          static_for<0, Q>([&](auto qc) {
+          // A target that is a wall itself holds nothing that is read (its slot q would carry the
+          // bounced population of x, a wall), so its store may carry any value.
+          const uint32_t tm = tmv[v];
+          static_for<1, Q>([&](auto qc) {
             constexpr int q = qc;
-            f[v][q] = Store<S>::ld(F + q * N + nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q]));
+            if ((tm >> q) & 1u) f[v][q] = Store<S>::ld(F + q * N + nbr_index(p, i, j, k, kCX[q], kCY[q], kCZ[q]));
           });
         }
       }

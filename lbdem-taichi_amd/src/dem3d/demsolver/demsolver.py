@@ -423,6 +423,10 @@ class DEMSolver:
         return dict(slot_overflow=int(s[0]), dt_warnings=int(s[1]), pp_overlap_warnings=int(s[2]),
                     pw_overlap_warnings=int(s[3]), pair_overflow=int(s[4]))
 
+    def reset_status(self):
+        """Clear the device anomaly counters (stream-ordered)."""
+        self._status.zero_()
+
     def check(self):
         """Raise if contacts were lost to a full history slot array or candidate list."""
         st = self.status()

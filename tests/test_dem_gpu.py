@@ -89,10 +89,14 @@ def test_dem_trajectories_match_oracle(model, path):
     assert gpu.status()["slot_overflow"] == 0
 
 
-def test_dem_coarse_grid_polydisperse_matches_oracle():
+@pytest.mark.parametrize("offset", [0.0, 1000.0])
+def test_dem_coarse_grid_polydisperse_matches_oracle(offset):
     """Two grains of R 0.13 / 0.11 among ~2,900 of R <= 0.02: the broad-phase cell (4 r_max)
     makes a 2 x 2 x 2 grid, so each particle has ~1,000 candidates and the multi-kernel probe
-    stages whole buckets in LDS.  Trajectories and contact sets follow the oracle."""
+    stages whole buckets in LDS.  Trajectories and contact sets follow the oracle.  With the
+    domain at [1000, 1001]^3 the staged probe's FP32 pre-test works on coordinates rounded by up
+    to 3e-5, the size of the contact overlaps, so only its rounding slack (2^-20 of the
+    coordinates' magnitude) keeps every touching pair."""
     rng = np.random.default_rng(3)
     box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     big = np.array([[0.3, 0.35, 0.4], [0.68, 0.62, 0.55]])
@@ -103,6 +107,8 @@ def test_dem_coarse_grid_polydisperse_matches_oracle():
     rad = np.concatenate([Rb, rad_s[keep]])
     mass = 2500.0 * 4 / 3 * np.pi * rad ** 3
     vel = np.vstack([np.zeros((2, 3)), vel_s[keep]])
+    pos = pos + offset
+    box = tuple(b + offset for b in box)
     cfg = make_config(model="hertz", box=box, dt=5e-5)
     gpu = gpu_solver(cfg, rad, mass, pos, vel, path=2)
     orc = oracle_solver(cfg, rad, mass, pos, vel)
