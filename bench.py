@@ -573,54 +573,70 @@ def precision_study(n=256, steps=20000, every=500, checkpoints=(1000, 5000, 1000
             "smagorinsky": {"Cs": 0.1, "mode": smag}, "results": out}
 
 
-def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=10, relax=4000, K=64):
+def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=10, relax=4000, K=64, tau=0.503):
     """Unresolved EqIBM LBM-DEM at its full single-GPU shape: 128^3 lattice (walls on all faces),
     200,000 spheres with log-normal diameters (mean 0.5 lu, sigma_log 0.2, clipped below 1 lu) in
-    a 64 x 64 x 9 lu bed on the floor (solid fraction 0.4, porosity 0.6), density ratio 2.5, soft
-    linear contacts (k_n = k_t = 1e-3, damping ratio 0.3, friction 0.5), broad phase with cells
+    a 64 x 64 x 9 lu bed on the floor (solid fraction 0.4, porosity 0.6), density ratio 2.5,
+    linear contacts (k_n = k_t = 1e-2, damping ratio 0.3, friction 0.5), broad phase with cells
     one maximum diameter wide and the 27-cell probe, 10 DEM substeps per lattice step, FP32
-    populations.  The seeded positions overlap; `relax` DEM substeps without gravity (not timed)
-    push the overlaps apart first, and the status counters are cleared after them, so the timed
-    steps start from a bed without large overlaps and must lose no contact (dem.check()).  A
+    populations, tau = 0.503 (the reference's many-grain settling example, sili_settle.py).
+
+    The fluid drag on a grain is computed once per lattice step and held over its DEM substeps
+    (eqlattice.py:309-386, demsolver.py:447), an explicit update that is stable while
+    beta dt / m < 2, with beta / m = 18 mu (1 - phi)^2 F(phi, Re) / (rho_p d^2): about 0.2 for the
+    mean grain at tau = 0.503, but 7.5 at tau = 0.6, where the grains' velocities grow without
+    bound and the positions turn to NaN within a few dozen steps.
+
+    The seeded positions overlap.  `relax` DEM substeps without gravity, with walls on the bed
+    box's six faces (not timed), push the overlaps apart at the bed's porosity; the relaxed bed
+    then starts a solver over the whole domain, whose status counters the timed steps fill.  A
     timed step is: DEM substeps (general path, gravity -1e-5 lu), collide-stream with the
     Tenneti weights, update_coupling (spreading, finalize, drag).  Reports per-phase times (HIP
-    events), the step's MLUPS and the DEM status counters of the timed steps."""
+    events), the step's MLUPS, the DEM status counters of the timed steps (dem.check()) and
+    whether every grain's position and velocity is finite at the end (the study raises if not)."""
     import numpy as np
     import torch
-    from src.dem3d.demconfig import DEMSolverConfig, DomainBounds, LinearContactConfig, Vector3
+    from src.dem3d.demconfig import DEMSolverConfig, DomainBounds, LinearContactConfig
     from src.dem3d.demsolver import DEMSolver
     from src.unlbdem.eqlattice import EqIMBlattice3D
     rng = np.random.default_rng(seed)
     d = np.minimum(np.exp(np.log(0.5) - 0.02 + 0.2 * rng.standard_normal(n)), 0.999)
     rad = 0.5 * d
     L = np.array([N - 2.0, N - 2.0, N - 2.0])
-    lo = np.array([(L[0] - 64) / 2, (L[1] - 64) / 2, 0.0])
-    pos = lo + rad[:, None] + rng.random((n, 3)) * (np.array([64.0, 64.0, 9.0]) - 2 * rad[:, None])
+    bed = np.array([64.0, 64.0, 9.0])
+    lo = np.array([(L[0] - bed[0]) / 2, (L[1] - bed[1]) / 2, 0.0])
+    pos = lo + rad[:, None] + rng.random((n, 3)) * (bed - 2 * rad[:, None])
     mass = 2.5 * 4.0 / 3.0 * np.pi * rad ** 3
-    cm = LinearContactConfig(stiffness_normal=1e-3, stiffness_tangential=1e-3, damping_normal=0.3,
+    cm = LinearContactConfig(stiffness_normal=1e-2, stiffness_tangential=1e-2, damping_normal=0.3,
                              damping_tangential=0.3, pp_friction=0.5, pw_friction=0.5)
-    cfg = DEMSolverConfig(domain=DomainBounds(0.0, L[0], 0.0, L[1], 0.0, L[2]), dt=1.0 / subcycles,
-                          gravity=(0.0, 0.0, 0.0), contact_model=cm)
-    cfg.set_particle_properties(elastic_modulus=1.0, poisson_ratio=0.3, max_coordinate_number=K)
-    cfg.set_wall_properties(elastic_modulus=1.0, poisson_ratio=0.3, density=2.5)
-    dem = DEMSolver(cfg)
-    dem.init_particle_arrays(radius=rad, mass=mass, position=pos, cell_size=float(d.max()), probe27=True)
-    dem.set_contact_model("linear")
+
+    def solver(box, gravity, position):
+        cfg = DEMSolverConfig(domain=DomainBounds(box[0], box[1], box[2], box[3], box[4], box[5]),
+                              dt=1.0 / subcycles, gravity=gravity, contact_model=cm)
+        cfg.set_particle_properties(elastic_modulus=1.0, poisson_ratio=0.3, max_coordinate_number=K)
+        cfg.set_wall_properties(elastic_modulus=1.0, poisson_ratio=0.3, density=2.5)
+        s = DEMSolver(cfg)
+        s.init_particle_arrays(radius=rad, mass=mass, position=position, cell_size=float(d.max()), probe27=True)
+        s.set_contact_model("linear")
+        return s
+
     t0 = time.perf_counter()
     relax_status = []
-    for r in range(0, relax, 500):
-        dem.run_substeps(min(500, relax - r))
-        torch.cuda.synchronize()
-        relax_status.append(dem.status()["pp_overlap_warnings"])
-        dem.reset_status()
-        print(f"unresolved study: relaxation {r + 500} substeps, overlap warnings in them {relax_status[-1]}",
-              file=sys.stderr, flush=True)
+    if relax > 0:
+        hi = lo + bed
+        rel = solver((lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]), (0.0, 0.0, 0.0), pos)
+        for r in range(0, relax, 500):
+            rel.run_substeps(min(500, relax - r))
+            torch.cuda.synchronize()
+            relax_status.append(rel.status()["pp_overlap_warnings"])
+            rel.reset_status()
+            print(f"unresolved study: relaxation {r + 500} substeps, overlap warnings in them {relax_status[-1]}",
+                  file=sys.stderr, flush=True)
+        pos = rel.gf.position.to_numpy()
+        del rel
     relax_s = time.perf_counter() - t0
-    dem.gf.velocity.fill(0.0)
-    dem.gf.omega.fill(0.0)
-    cfg.gravity = Vector3(0.0, 0.0, -1e-5)
-    dem.invalidate()
-    lat = EqIMBlattice3D(N, N, N, 1.0 / 0.6, 1.0, 1.0, 1.0, dem)
+    dem = solver((0.0, L[0], 0.0, L[1], 0.0, L[2]), (0.0, 0.0, -1e-5), pos)
+    lat = EqIMBlattice3D(N, N, N, 1.0 / tau, 1.0, 1.0, 1.0, dem)
     lat.CT.from_numpy(walls_ct(N, N, N))
     lat.initialize_complete()
 
@@ -657,8 +673,14 @@ def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=
     ms = e0.elapsed_time(e1) / steps
     per = {k: round(sum(a.elapsed_time(b) for a, b in v) / len(v), 4) for k, v in ev.items()}
     st = dem.check()
-    return {"study": "unresolved", "lattice": N, "particles": n, "steps": steps, "subcycles": subcycles,
-            "bed": "64 x 64 x 9 lu, solid fraction 0.40", "relaxation_substeps": relax,
+    p_end, v_end = dem.gf.position.to_numpy(), dem.gf.velocity.to_numpy()
+    finite = bool(np.isfinite(p_end).all() and np.isfinite(v_end).all())
+    if not finite:
+        raise RuntimeError("unresolved study: grain positions or velocities are not finite after the timed steps")
+    return {"study": "unresolved", "lattice": N, "particles": n, "steps": steps, "subcycles": subcycles, "tau": tau,
+            "bed": "64 x 64 x 9 lu, solid fraction 0.40", "contacts": "linear, k_n = k_t = 1e-2, damping 0.3",
+            "relaxation_substeps": relax, "grains_finite": finite, "grain_speed_max": float(np.abs(v_end).max()),
+            "bed_top": float(p_end[:, 2].max()),
             "relaxation_s": round(relax_s, 2), "relaxation_overlap_warnings_per_500": relax_status,
             "ms_per_step": round(ms, 4), "mlups": round(N ** 3 / ms / 1e3, 1), "phase_ms": per,
             "dem_substep_ms": round(per["dem"] / subcycles, 4),

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 5
+#define LBDEM_ABI_VERSION 6
 
 /* return codes */
 #define LBDEM_OK 0
@@ -408,6 +408,10 @@ typedef struct lbdem_dem_work {
                                         path keeps between calls (n <= 256); zero it, or its first
                                         element, whenever radii or contact history change */
   double *verlet_ref;                /* 1 + 3n, may be NULL (with verlet) */
+  double *spos;                      /* 4n, may be NULL: x, y, z and radius of each particle in
+                                        bucket order, written by the general path of lbdem_dem_run
+                                        so that its probe walks particles and candidates in space
+                                        order (NULL: particle order) */
 } lbdem_dem_work;
 
 /* Candidate pairs of the Morton-hash broad phase: BPCD.detect_collision with the count,

@@ -23,7 +23,7 @@ class DemState(ctypes.Structure):
 class DemWork(ctypes.Structure):
     _fields_ = [("count", _vp), ("offset", _vp), ("current", _vp), ("tile_sum", _vp), ("hash", _vp),
                 ("pid", _vp), ("pairs", _vp), ("pair_cap", ctypes.c_int64), ("npairs", _vp),
-                ("need_new", _vp), ("verlet", _vp), ("verlet_ref", _vp)]
+                ("need_new", _vp), ("verlet", _vp), ("verlet_ref", _vp), ("spos", _vp)]
 
 
 P = ctypes.POINTER
