@@ -563,31 +563,19 @@ __device__ __forceinline__ void stage_pair(int i, int j, int2 *s_pairs, int *s_n
 template <int NP>
 __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P, const double *pos,
                                                         const double *rad, const int32_t *count,
-                                                        const int32_t *offset, const int32_t *pid,
-                                                        const double4 *spos, int32_t *pairs, int64_t cap,
-                                                        unsigned long long *npairs) {
-  // with spos, lane t takes the t-th particle in bucket order, so a wavefront's particles and
-  // their candidates lie close together in space and in spos
-  const int t0 = blockIdx.x * 256 + threadIdx.x;
-  const bool active = t0 < P.n;
-  const int i = (spos && active) ? pid[t0] : t0;
+                                                        const int32_t *offset, const int32_t *pid, int32_t *pairs,
+                                                        int64_t cap, unsigned long long *npairs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool active = i < P.n;
   const int split = gridDim.y, part = blockIdx.y;
   int32_t h[NP];
   double xi[3] = {0.0, 0.0, 0.0}, ri = 0.0;
 #pragma unroll
   for (int k = 0; k < NP; ++k) h[k] = -1;
   if (active) {
-    if (spos) {
-      const double4 q = spos[t0];
-      xi[0] = q.x;
-      xi[1] = q.y;
-      xi[2] = q.z;
-      ri = q.w;
-    } else {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) xi[a] = pos[3 * i + a];
-      ri = rad[i];
-    }
+    for (int a = 0; a < 3; ++a) xi[a] = pos[3 * i + a];
+    ri = rad[i];
     int32_t c[3];
     cell_of(P, xi, c);
     probe_cells<NP>(P, xi, c, h);
@@ -664,19 +652,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
         const int t = base + (int)threadIdx.x;
         if (t < e) {
           const int32_t j = pid[t];
-          double x, y, z, r;
-          if (spos) {
-            const double4 q = spos[t];
-            x = q.x;
-            y = q.y;
-            z = q.z;
-            r = q.w;
-          } else {
-            x = pos[3 * j];
-            y = pos[3 * j + 1];
-            z = pos[3 * j + 2];
-            r = rad[j];
-          }
+          const double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2], r = rad[j];
           s_j[threadIdx.x] = j;
           s_x[threadIdx.x] = x;
           s_y[threadIdx.x] = y;
@@ -712,16 +688,69 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
       const int32_t b = offset[h[k]], e = b + count[h[k]];
       for (int idx = b + part; idx < e; idx += split) {
         const int32_t j = pid[idx];
-        if (j <= i) continue;
-        bool touch;
-        if (spos) {
-          const double4 q = spos[idx];
-          const double xj[3] = {q.x, q.y, q.z};
-          touch = touching(P, xi, ri, xj, q.w);
-        } else {
-          touch = touching(P, xi, ri, pos + 3 * j, rad[j]);
+        if (j <= i || !touching(P, xi, ri, pos + 3 * j, rad[j])) continue;
+        stage_pair<kStage>(i, j, s_pairs, &s_np, pairs, cap, npairs);
+      }
+    }
+  }
+  __syncthreads();
+  const int ns = min(s_np, kStage);
+  if (threadIdx.x == 0) s_base = ns > 0 ? atomicAdd(npairs, (unsigned long long)ns) : 0ull;
+  __syncthreads();
+  const unsigned long long base = s_base;
+  for (int u = threadIdx.x; u < ns; u += 256) {
+    const unsigned long long w = base + (unsigned long long)u;
+    if (w < (unsigned long long)cap) reinterpret_cast<int2 *>(pairs)[w] = s_pairs[u];
+  }
+}
+
+// The touching probe of the 27-cell rule over particles in bucket order (spos): lane t takes
+// the t-th particle of the bucket order, so a wavefront's particles share most of their buckets
+// and read their candidates from neighbouring 32-byte records.  The walk is latency-bound (a
+// bucket's header, then its entries), so blockIdx.y splits each particle's 27 buckets, not
+// their entries, over gridDim.y workgroups, and a bucket's entries are loaded four at a time:
+// one chain of two dependent loads per bucket and a few buckets per lane.
+__global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_params P, const int32_t *count,
+                                                               const int32_t *offset, const int32_t *pid,
+                                                               const double4 *spos, int32_t *pairs, int64_t cap,
+                                                               unsigned long long *npairs) {
+  constexpr int kStage = pair_stage_size<27>();
+  __shared__ int2 s_pairs[kStage];
+  __shared__ int s_np;
+  __shared__ unsigned long long s_base;
+  if (threadIdx.x == 0) s_np = 0;
+  __syncthreads();
+  const int t0 = blockIdx.x * 256 + threadIdx.x;
+  const int split = gridDim.y, part = blockIdx.y;
+  if (t0 < P.n) {
+    const int i = pid[t0];
+    const double4 q = spos[t0];
+    const double xi[3] = {q.x, q.y, q.z};
+    const double ri = q.w;
+    int32_t c[3], h[27];
+    cell_of(P, xi, c);
+    probe_cells<27>(P, xi, c, h);
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+      if (k % split != part || h[k] < 0) continue;
+      const int32_t b = offset[h[k]], e = b + count[h[k]];
+      for (int idx = b; idx < e; idx += 4) {
+        int32_t jj[4];
+        double4 qq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          jj[u] = -1;
+          if (idx + u < e) {
+            jj[u] = pid[idx + u];
+            qq[u] = spos[idx + u];
+          }
         }
-        if (touch) stage_pair<kStage>(i, j, s_pairs, &s_np, pairs, cap, npairs);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (jj[u] <= i) continue;
+          const double xj[3] = {qq[u].x, qq[u].y, qq[u].z};
+          if (touching(P, xi, ri, xj, qq[u].w)) stage_pair<kStage>(i, jj[u], s_pairs, &s_np, pairs, cap, npairs);
+        }
       }
     }
   }
@@ -1400,18 +1429,23 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
     // there are fewer than 2048 workgroups (more waves in flight; also measured faster on the
     // sparse 200,000-particle bed, 0.72 against 0.84 ms with no split)
     const unsigned split = (unsigned)std::min(16, std::max(1, (int)((2048 + nb - 1) / nb)));
-    // bucket order for the 27-cell probe (fine grids: a wavefront's particles then share buckets
-    // and read their candidates from neighbouring records; 0.54 -> 0.20 ms per substep on the
-    // 200,000-grain bed).  The 8-cell probe serves coarse grids, whose workgroups mostly take
-    // the LDS-tiled walk; there the bucket order measured slower (0.70 -> 0.86 ms, 37,464 grains)
+    // the 27-cell probe (fine grids) walks particles in bucket order (200,000-grain bed: 0.20
+    // ms per substep with a per-entry split, 0.54 ms in particle order; a walk of each
+    // wavefront's box of cells together, one candidate per lane and readlane, measured 0.47 ms:
+    // waves at jumps of the Morton order get boxes of hundreds of cells and set the launch's
+    // tail).  The 8-cell probe serves coarse grids, whose workgroups mostly take its LDS-tiled
+    // walk; bucket order measured slower there (0.70 -> 0.86 ms, 37,464 grains)
     const double4 *sp = P->probe27 ? reinterpret_cast<const double4 *>(w->spos) : nullptr;
-    if (sp) hipLaunchKernelGGL(k_bp_sort_state, dim3(nb), dim3(256), 0, s, *P, pos, rad_touch, w->pid, w->spos);
-    if (P->probe27)
-      hipLaunchKernelGGL(k_bp_probe_touch<27>, dim3(nb, split), dim3(256), 0, s, *P, pos, rad_touch, w->count,
+    if (sp) {
+      hipLaunchKernelGGL(k_bp_sort_state, dim3(nb), dim3(256), 0, s, *P, pos, rad_touch, w->pid, w->spos);
+      hipLaunchKernelGGL(k_bp_probe_touch_sorted, dim3(nb, std::min(split, 27u)), dim3(256), 0, s, *P, w->count,
                          w->offset, w->pid, sp, w->pairs, w->pair_cap, w->npairs);
+    } else if (P->probe27)
+      hipLaunchKernelGGL(k_bp_probe_touch<27>, dim3(nb, split), dim3(256), 0, s, *P, pos, rad_touch, w->count,
+                         w->offset, w->pid, w->pairs, w->pair_cap, w->npairs);
     else
       hipLaunchKernelGGL(k_bp_probe_touch<8>, dim3(nb, split), dim3(256), 0, s, *P, pos, rad_touch, w->count,
-                         w->offset, w->pid, sp, w->pairs, w->pair_cap, w->npairs);
+                         w->offset, w->pid, w->pairs, w->pair_cap, w->npairs);
   } else if (P->probe27) {
     hipLaunchKernelGGL(k_bp_probe<27>, dim3(nb), dim3(256), 0, s, *P, pos, w->count, w->offset, w->pid, w->pairs,
                        w->pair_cap, w->npairs);

@@ -126,6 +126,56 @@ def test_dem_coarse_grid_polydisperse_matches_oracle(offset):
     gpu.check()
 
 
+def _dense_bed(n, ext, seed):
+    """Grains of log-normal diameter (mean 0.5, sigma_log 0.2, below 1) seeded uniformly, with
+    overlaps, in a box of extents ext (the unresolved bed's size law)."""
+    rng = np.random.default_rng(seed)
+    d = np.minimum(np.exp(np.log(0.5) - 0.02 + 0.2 * rng.standard_normal(n)), 0.999)
+    rad = 0.5 * d
+    pos = rad[:, None] + rng.random((n, 3)) * (np.array(ext) - 2 * rad[:, None])
+    return rad, 2.5 * 4.0 / 3.0 * np.pi * rad ** 3, pos
+
+
+def _boxed_wave_fraction(pos, cell, T):
+    """Share of 64-particle wavefronts (particles in bucket order) whose cells lie within a box of
+    at most 343 cells, i.e. that the touching probe walks together."""
+    c = np.floor(pos / cell).astype(np.int64)
+    h = np.array([O.morton3d32(*v) for v in c]) & (T - 1)
+    c = c[np.argsort(h, kind="stable")]
+    boxed = [np.prod(c[w:w + 64].max(0) - c[w:w + 64].min(0) + 3) <= 343 for w in range(0, len(c), 64)]
+    return float(np.mean(boxed))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_dem_fine_grid_probe27_matches_oracle(periodic):
+    """6,000 grains seeded overlapping in a 12 x 12 x 6 box (about 7 per cell), broad-phase cells
+    one maximum diameter wide with the 27-cell probe, as in the unresolved bed: most wavefronts
+    of the general path's touching probe walk their box of cells together, the rest per lane.
+    100 substeps of soft linear contacts (and a periodic x axis) against the oracle: positions,
+    velocities and the contact sets."""
+    rad, mass, pos = _dense_bed(6000, (12.0, 12.0, 6.0), seed=21)
+    vel = np.zeros_like(pos)
+    cell = float(2.0 * rad.max())
+    cfg = make_config(model="linear", box=(0.0, 12.0, 0.0, 12.0, 0.0, 6.0), dt=0.1, g=(0.0, 0.0, -1e-5), K=64,
+                      periodic=(periodic, False, False), kn=1e-2, kt=1e-2, zeta=0.3, mu=0.5)
+    gpu = gpu_solver(cfg, rad, mass, pos, vel, path=2, cell_size=cell, probe27=True)
+    orc = oracle_solver(cfg, rad, mass, pos, vel, cell=cell, probe27=True)
+    assert _boxed_wave_fraction(pos, cell, orc.params.T) > 0.5
+    for _ in range(2):
+        gpu.run_substeps(50)
+        orc.step(50)
+    torch.cuda.synchronize()
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+    assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
+    K = gpu.K
+    act = gpu._cf_active.cpu().numpy().astype(bool)
+    got = sorted(zip(np.repeat(np.arange(gpu.n), K)[act].tolist(), gpu._cf_j.cpu().numpy()[act].tolist()))
+    oact = orc["cf_active"].astype(bool)
+    ref = sorted(zip(np.repeat(np.arange(orc.n), K)[oact].tolist(), orc["cf_j"][oact].tolist()))
+    assert got == ref and len(ref) > 10000
+    gpu.check()
+
+
 def test_dem_anisotropic_inertia_matches_oracle():
     """Full inertia tensors (principal moments 1 : 1.4 : 0.7, randomly rotated): the general
     rotation update (body frame, inverse inertia, Euler term), where spheres take the isotropic
