@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 6
+#define LBDEM_ABI_VERSION 7
 
 /* return codes */
 #define LBDEM_OK 0
@@ -242,6 +242,10 @@ typedef struct lbdem_grains {
   const double *radius;      /* n */
   double *force_fluid;       /* 3n, written by the couplings */
   double *moment_fluid;      /* 3n */
+  const int32_t *order;      /* n, may be NULL: a permutation of 0..n-1 that the EqIBM spreading
+                                and drag visit the grains in (grains close in space close in the
+                                order, e.g. the DEM broad phase's bucket order); other couplings
+                                ignore it */
 } lbdem_grains;
 
 /* lattice <-> physical units (UnitConverter, src/lbm3d/lbmutils.py:14-63) and the DEM

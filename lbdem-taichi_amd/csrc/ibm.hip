@@ -23,6 +23,7 @@ struct IbmArgs {
   double *force_out, *moment_out;
   int32_t *status;
   const uint32_t *key;  // hybrid coupling: coarse-grain keys of the PSC mapping
+  const int32_t *order;  // EqIBM: grains visited in this order (a permutation), or NULL
 };
 
 // EqIMBlattice3D.threedelta (src/unlbdem/eqlattice.py:451-462)
@@ -117,6 +118,94 @@ __global__ __launch_bounds__(256) void k_eq_spread(const IbmArgs a) {
   unsafeAtomicAdd(a.wsum + c, (float)wn);
 }
 
+// The same passes over grains in a spatial order (IbmArgs::order), kSpreadGrains per workgroup:
+// the workgroup accumulates its grains' contributions in LDS over the union of their boxes and
+// adds each covered cell to the global fields once.  In particle order every grain adds ~14
+// cells x 5 fields with global atomics (14 million per step for 200,000 grains, 0.46 ms); in a
+// spatial order without the LDS stage, concurrent wavefronts hit the same cells and it is
+// slower still (0.63 ms).  A workgroup whose union box exceeds kSpreadCells adds directly.
+constexpr int kSpreadGrains = 128;
+constexpr int kSpreadCells = 1536;
+
+__global__ __launch_bounds__(256) void k_eq_spread_ordered(const IbmArgs a) {
+  __shared__ float s_acc[5][kSpreadCells];
+  __shared__ int s_lo[3], s_hi[3];
+  const int g0 = blockIdx.x * kSpreadGrains;
+  const int ng = min(kSpreadGrains, a.np - g0);
+  if (threadIdx.x < 3) {
+    s_lo[threadIdx.x] = INT_MAX;
+    s_hi[threadIdx.x] = INT_MIN;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < ng) {
+    double c[3];
+    lattice_pos(a, a.order[g0 + threadIdx.x], c[0], c[1], c[2]);
+    const int N[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int lo, hi;
+      eq_box(c[d], N[d], lo, hi);
+      if (hi > lo) {
+        atomicMin(&s_lo[d], lo);
+        atomicMax(&s_hi[d], hi);
+      }
+    }
+  }
+  __syncthreads();
+  const int lo0 = s_lo[0], lo1 = s_lo[1], lo2 = s_lo[2];
+  const int64_t e0 = (int64_t)s_hi[0] - lo0, e1 = (int64_t)s_hi[1] - lo1, e2 = (int64_t)s_hi[2] - lo2;
+  const bool any = e0 > 0 && e1 > 0 && e2 > 0;
+  const bool lds = any && e0 * e1 * e2 <= kSpreadCells;
+  const int vol = lds ? (int)(e0 * e1 * e2) : 0, ex1 = (int)e1, ex2 = (int)e2;
+  for (int l = threadIdx.x; l < vol; l += 256)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) s_acc[m][l] = 0.f;
+  __syncthreads();
+  const double pi = 3.14159265358979323846;
+  const double Vl = a.dx * a.dx * a.dx, s = a.dt / a.dx;
+  if (any) {
+    for (int q = (int)(threadIdx.x >> 6); q < ng; q += 4) {  // uniform over the wavefront
+      const int p = a.order[g0 + q];
+      double xc, yc, zc;
+      lattice_pos(a, p, xc, yc, zc);
+      const BoxLane b = box_lane(a, xc, yc, zc);
+      double wb = -1.0;
+      if (b.in && (a.flags[b.c] & LBDEM_FLAG_ACTIVE)) wb = kernel_mirror(a, xc, yc, zc, b.i, b.j, b.k);
+      const double denom = wave_sum(wb >= 0.0 ? wb : 0.0);
+      if (denom < 1e-30 || wb < 0.0) continue;
+      const double R = a.rad[p];
+      const double Vp = 4.0 / 3.0 * pi * R * R * R;
+      const double wn = wb / denom, e = wn * Vp / Vl;
+      if (e == 0.0) continue;
+      const float v[5] = {(float)e, (float)(a.vel[3 * p] * s * e), (float)(a.vel[3 * p + 1] * s * e),
+                          (float)(a.vel[3 * p + 2] * s * e), (float)wn};
+      if (lds) {
+        const int l = ((b.i - lo0) * ex1 + (b.j - lo1)) * ex2 + (b.k - lo2);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) atomicAdd(&s_acc[m][l], v[m]);
+      } else {
+        const int c = b.c;
+        unsafeAtomicAdd(a.volfrac + c, v[0]);
+        unsafeAtomicAdd(a.velsum + 3 * c, v[1]);
+        unsafeAtomicAdd(a.velsum + 3 * c + 1, v[2]);
+        unsafeAtomicAdd(a.velsum + 3 * c + 2, v[3]);
+        unsafeAtomicAdd(a.wsum + c, v[4]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < vol; l += 256) {
+    if (s_acc[0][l] == 0.f) continue;  // e > 0 for every contribution: 0 means none
+    const int i = lo0 + l / (ex1 * ex2), j = lo1 + (l / ex2) % ex1, k = lo2 + l % ex2;
+    const int c = (i * a.ny + j) * a.nz + k;
+    unsafeAtomicAdd(a.volfrac + c, s_acc[0][l]);
+    unsafeAtomicAdd(a.velsum + 3 * c, s_acc[1][l]);
+    unsafeAtomicAdd(a.velsum + 3 * c + 1, s_acc[2][l]);
+    unsafeAtomicAdd(a.velsum + 3 * c + 2, s_acc[3][l]);
+    unsafeAtomicAdd(a.wsum + c, s_acc[4][l]);
+  }
+}
+
 // solid-fraction clamp and solid velocity (eqlattice.py:188-194); fluid rho, u of the
 // current populations at covered cells (the `vel` that compute_weight and lattice2grains read)
 template <typename S>
@@ -175,8 +264,11 @@ __device__ __forceinline__ double tenneti_cd(double Re, double svf, double lim) 
 // particle (measured faster than a wavefront per particle with six FP64 wave reductions:
 // 88 against 209 us for 200,000 particles)
 __global__ __launch_bounds__(256) void k_eq_drag(const IbmArgs a) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= a.np) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.np) return;
+  // in a spatial order neighbouring threads read neighbouring cells (68 against 128 us for
+  // 200,000 grains)
+  const int p = a.order ? a.order[t] : t;
   double xc, yc, zc;
   lattice_pos(a, p, xc, yc, zc);
   int i0, i1, j0, j1, k0, k1;
@@ -500,6 +592,7 @@ int fill_ibm(const lbdem_lattice *lat, const uint8_t *flags, const lbdem_grains 
   a.dmin1 = u->dmin[1];
   a.dmin2 = u->dmin[2];
   a.nu_lu = nu_lu;
+  a.order = g->order;
   return LBDEM_OK;
 }
 unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -527,7 +620,9 @@ int lbdem_ibm_spread(const lbdem_lattice *lat, const lbdem_step_args *st, const 
   hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, volfrac, (size_t)n);
   hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, velsum, (size_t)(3 * n));
   hipLaunchKernelGGL(k_fill_zero, dim3(2048), dim3(256), 0, s, wsum, (size_t)n);
-  if (g->n > 0)
+  if (g->n > 0 && a.order)
+    hipLaunchKernelGGL(k_eq_spread_ordered, dim3((g->n + kSpreadGrains - 1) / kSpreadGrains), dim3(256), 0, s, a);
+  else if (g->n > 0)
     hipLaunchKernelGGL(k_eq_spread, dim3((g->n + kGrainsPerBlock - 1) / kGrainsPerBlock), dim3(256), 0, s, a);
   LBDEM_CHECK_LAUNCH("lbdem_ibm_spread");
   // finalize reads the populations for the fluid velocity at covered cells

@@ -18,6 +18,10 @@ from ... import _hip
 from ..._field import Field
 from ...runtime import config
 from .. import _abi
+
+# lbdem_dem_run's automatic choice takes the single-workgroup path up to this many particles
+# (kSmallN in csrc/dem.hip)
+SMALL_PATH_MAX = 2048
 from ..bpcd import BPCD
 from ..contactmanager import HertzMindlinContactModel, LinearContactModel
 from ..demconfig import DEMSolverConfig, HertzContactConfig, LinearContactConfig
@@ -324,6 +328,8 @@ class DEMSolver:
         self._verlet = torch.zeros(2 + 17 * n, dtype=torch.int32, device=self.device)
         self._verlet_ref = torch.zeros(1 + 3 * n, dtype=torch.float64, device=self.device)
         self._radius_version = None
+        self._cached = None
+        self._order_valid = False
 
     def set_contact_model(self, model_type: str = "hertz"):
         """Select the contact law; must match the config's contact model (demsolver.py:311-347)."""
@@ -410,8 +416,20 @@ class DEMSolver:
             self._verlet[:1].zero_()
         _hip.check(_hip.lib().lbdem_dem_run(cached[3], cached[4], cached[5], int(nsub), int(self.path),
                                             _hip.stream()), "lbdem_dem_run")
+        # the grid-wide path leaves the broad phase's bucket order in the workspace's pid
+        self._order_valid = self.path == 2 or (self.path == 0 and self.n > SMALL_PATH_MAX)
         self.gf.touch(skip=_DEM_READ_ONLY)
         self._radius_version = self.gf.radius.version
+
+    def spatial_order(self):
+        """The particle indices in broad-phase bucket order of the last grid-wide substep (an
+        int32 permutation of 0..n-1, grains close in space close in the order), or None when no
+        substep has run on the grid-wide path since the arrays were set up.  The unresolved
+        couplings visit grains in this order for locality."""
+        cached = getattr(self, "_cached", None)
+        if not getattr(self, "_order_valid", False) or cached is None:
+            return None
+        return cached[6]["pid"]
 
     def run_simulation(self):
         """One DEM step (demsolver.py:848-862)."""

@@ -24,7 +24,7 @@ _vp = ctypes.c_void_p
 
 class Grains(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("position", _vp), ("velocity", _vp), ("omega", _vp), ("radius", _vp),
-                ("force_fluid", _vp), ("moment_fluid", _vp)]
+                ("force_fluid", _vp), ("moment_fluid", _vp), ("order", _vp)]
 
 
 class Units(ctypes.Structure):

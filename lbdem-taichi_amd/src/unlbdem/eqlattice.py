@@ -54,6 +54,9 @@ class EqIMBlattice3D(BasicLattice3D):
             self._gs = GrainStructs(self)
         else:
             self.dem.gf.sync()
+        # spreading and drag visit the grains in the DEM's bucket order when there is one
+        order = self.dem.spatial_order()
+        self._gs.g.order = order.data_ptr() if order is not None else None
         return self._gs
 
     def grains2lattice(self):

@@ -40,12 +40,12 @@ def _walls(CT):
     CT[:, :, 0], CT[:, :, -1] = O.OBSTACLE, O.OBSTACLE
 
 
-def _setup(cls, kind, N=20, tau=0.6, cs=0.17, cluster=False):
+def _setup(cls, kind, N=20, tau=0.6, cs=0.17, cluster=False, particles=None, path=0):
     from src.lbm3d.lbm_solver3d import BasicLattice3D
-    pos, vel, rad, mass = _particles(N=N, cluster=cluster)
+    pos, vel, rad, mass = _particles(N=N, cluster=cluster) if particles is None else particles
     box = (0.0, (N - 1) * DX, 0.0, (N - 1) * DX, 0.0, (N - 1) * DX)
     cfg = make_config(box=box, dt=DT / 10, g=(0, 0, 0))
-    dem = gpu_solver(cfg, rad, mass, pos, vel)
+    dem = gpu_solver(cfg, rad, mass, pos, vel, path=path)
     BasicLattice3D.SmagorinskyConstant = cs
     L = cls(N, N, N, 1.0 / tau, DX, DT, RHO, dem)
     orc = O.OracleLattice(N, N, N, 1.0 / tau, kind=kind, cs=cs)
@@ -88,6 +88,65 @@ def test_eqibm_matches_oracle():
         assert np.abs(L.vel.to_numpy()[fluid] - orc.vel[fluid]).max() < 2e-4 * np.abs(orc.vel[fluid]).max()
         Fg = dem.gf.force_fluid.to_numpy()
         assert np.abs(Fg - F).max() < 5e-4 * np.abs(F).max()
+    finally:
+        BasicLattice3D.SmagorinskyConstant = old
+
+
+@pytest.mark.parametrize("layout", ["dense", "scattered"])
+def test_eqibm_in_bucket_order_matches_oracle(layout):
+    """After a grid-wide DEM substep the EqIBM spreading and drag visit the grains in the DEM's
+    bucket order (DEMSolver.spatial_order).  'dense' puts 600 grains into 8^3 lattice cells, so
+    most workgroups' unions of support boxes fit their LDS accumulator; 'scattered' spreads them
+    over the lattice, so workgroups add to the fields directly.  Mapping, weights and drag at the
+    DEM's positions, then the fluid velocity and the drag after 10 coupled steps, against the
+    oracle."""
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    from src.unlbdem.eqlattice import EqIMBlattice3D
+    N, n = 20, 600
+    rng = np.random.default_rng(12)
+    lo, hi = (5.0, 13.0) if layout == "dense" else (1.0, N - 2.0)
+    lat, rl = [], []
+    while len(lat) < n:  # no overlaps: the DEM substep must not turn contacts into large velocities
+        x, r = rng.uniform(lo, hi, 3), rng.uniform(0.2, 0.35)
+        if lat and (np.linalg.norm(np.array(lat) - x, axis=1) < np.array(rl) + r + 0.02).any():
+            continue
+        lat.append(x)
+        rl.append(r)
+    rad = np.array(rl) * DX
+    parts = ((np.array(lat) - 0.5) * DX, rng.uniform(-0.02, 0.02, (n, 3)) * DX / DT, rad,
+             2500.0 * 4 / 3 * np.pi * rad ** 3)
+    old = BasicLattice3D.SmagorinskyConstant
+    try:
+        L, orc, dem, _, _, _ = _setup(EqIMBlattice3D, "eqibm", N=N, particles=parts, path=2)
+        assert dem.spatial_order() is None
+        dem.run_substeps(1)
+        order = dem.spatial_order()
+        assert order is not None and np.array_equal(np.sort(order.cpu().numpy()), np.arange(n))
+        g = np.column_stack([dem.gf.position.to_numpy(), dem.gf.velocity.to_numpy(), rad])
+        L.initialize_complete()
+        orc.initialize()
+        O.ibm_spread(orc, g, [0, 0, 0], DX, DT)
+        O.ibm_weight(orc, DX, DT, RHO)
+        assert np.abs(L.volfrac.to_numpy() - orc.volfrac).max() < 2e-6
+        cov = orc.volfrac > 0
+        assert np.abs(L.velsolid.to_numpy()[cov] - orc.velsolid[cov]).max() < 1e-5 * np.abs(orc.velsolid).max()
+        assert np.abs(L.weight.to_numpy()[cov] - orc.weight[cov]).max() < 2e-4 * np.abs(orc.weight).max()
+        F = O.ibm_drag(orc, g, [0, 0, 0], DX, DT, RHO)
+        L.update_coupling()
+        assert np.abs(dem.gf.force_fluid.to_numpy() - F).max() < 5e-4 * np.abs(F).max()
+        for _ in range(10):
+            orc.collide()
+            orc.stream()
+            O.ibm_spread(orc, g, [0, 0, 0], DX, DT)
+            O.ibm_weight(orc, DX, DT, RHO)
+            F = O.ibm_drag(orc, g, [0, 0, 0], DX, DT, RHO)
+            L.collide()
+            L.stream()
+            L.update_coupling()
+        fluid = orc.CT == O.FLUID
+        assert np.abs(orc.vel[fluid]).max() < 0.05, "the oracle trajectory must stay stable"
+        assert np.abs(L.vel.to_numpy()[fluid] - orc.vel[fluid]).max() < 2e-4 * np.abs(orc.vel[fluid]).max()
+        assert np.abs(dem.gf.force_fluid.to_numpy() - F).max() < 5e-4 * np.abs(F).max()
     finally:
         BasicLattice3D.SmagorinskyConstant = old
 
