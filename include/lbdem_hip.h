@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 7
+#define LBDEM_ABI_VERSION 8
 
 /* return codes */
 #define LBDEM_OK 0
@@ -371,6 +371,17 @@ typedef struct lbdem_dem_params {
   double wall_n[6][3];  /* outward wall normals (DEMSolverConfig._setup_walls) */
   double wall_d[6];     /* n.x = d */
   int32_t wall_type[6]; /* 0 wall, 1 periodic (skipped) */
+  /* No counterpart in the reference (a search strategy, not a rule): with fine_cell > 0 the
+   * grid-wide path of lbdem_dem_run finds its touching pairs on a second grid of that cell size
+   * (27-cell probe) for the particles of radius <= large_r, and tests the nlarge particles of
+   * lbdem_dem_work.large (radius > large_r) against all particles; every pair found is kept
+   * only if it is a candidate of the grid above (j's bucket among i's probed buckets), so the
+   * pairs are those of the 8-cell probe.  Requires fine_cell > 2 large_r, probe27 = 0 and no
+   * periodic axis.  For strongly polydisperse packings, whose cell (4 r_max) holds most of the
+   * particles in a few buckets. */
+  double fine_cell;
+  double large_r;
+  int32_t nlarge;
 } lbdem_dem_params;
 
 /* Particle and contact state, device pointers (FP64 unless noted).  The Grain / Contact
@@ -416,6 +427,9 @@ typedef struct lbdem_dem_work {
                                         bucket order, written by the general path of lbdem_dem_run
                                         so that its probe walks particles and candidates in space
                                         order (NULL: particle order) */
+  int32_t *fhash;                    /* n, with fine_cell > 0: cell hashes on the second grid */
+  const int32_t *large;              /* nlarge, with fine_cell > 0: indices of the particles of
+                                        radius > large_r */
 } lbdem_dem_work;
 
 /* Candidate pairs of the Morton-hash broad phase: BPCD.detect_collision with the count,

@@ -765,6 +765,123 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_p
   }
 }
 
+// The touching pairs of the 8-cell rule on strongly polydisperse packings (lbdem_dem_params::
+// fine_cell > 0).  The grid of the rule (cell 4 r_max) then puts most particles into a few
+// buckets and its probe tests O(n^2) candidates.  Every touching pair lies within one cell of
+// that grid's octant block, so the touching pairs among the candidates are the touching pairs
+// that pass the candidate rule; they are found here on a second grid (fine_cell > 2 large_r,
+// 27-cell probe, particles in bucket order as in k_bp_probe_touch_sorted) for the particles of
+// radius <= large_r, and by testing each larger particle against all particles
+// (k_bp_touch_large), and a pair (i, j), i < j, is kept when j's bucket (hash of the rule's grid)
+// is one of i's probed buckets.
+__device__ __forceinline__ bool probed_by(const lbdem_dem_params &P, const double *xi, int32_t hj) {
+  int32_t c[3], h[8];
+  cell_of(P, xi, c);
+  probe_cells<8>(P, xi, c, h);
+  bool in = false;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) in |= (h[k] == hj);
+  return in;
+}
+
+__global__ __launch_bounds__(256) void k_bp_probe_touch_fine(const lbdem_dem_params P, const lbdem_dem_params Pf,
+                                                             const int32_t *count, const int32_t *offset,
+                                                             const int32_t *pid, const double4 *spos,
+                                                             const int32_t *hash, int32_t *pairs, int64_t cap,
+                                                             unsigned long long *npairs) {
+  constexpr int kStage = pair_stage_size<8>();
+  __shared__ int2 s_pairs[kStage];
+  __shared__ int s_np;
+  __shared__ unsigned long long s_base;
+  if (threadIdx.x == 0) s_np = 0;
+  __syncthreads();
+  const int t0 = blockIdx.x * 256 + threadIdx.x;
+  const int split = gridDim.y, part = blockIdx.y;
+  if (t0 < P.n) {
+    const int i = pid[t0];
+    const double4 q = spos[t0];
+    const double xi[3] = {q.x, q.y, q.z};
+    const double ri = q.w;
+    if (ri <= P.large_r) {
+      int32_t c[3], h[27], hm[8];
+      cell_of(Pf, xi, c);
+      probe_cells<27>(Pf, xi, c, h);
+      {
+        int32_t cm[3];
+        cell_of(P, xi, cm);
+        probe_cells<8>(P, xi, cm, hm);
+      }
+#pragma unroll
+      for (int k = 0; k < 27; ++k) {
+        if (k % split != part || h[k] < 0) continue;
+        const int32_t b = offset[h[k]], e = b + count[h[k]];
+        for (int idx = b; idx < e; idx += 4) {
+          int32_t jj[4];
+          double4 qq[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            jj[u] = -1;
+            if (idx + u < e) {
+              jj[u] = pid[idx + u];
+              qq[u] = spos[idx + u];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (jj[u] <= i || qq[u].w > P.large_r) continue;
+            const double xj[3] = {qq[u].x, qq[u].y, qq[u].z};
+            if (!touching(P, xi, ri, xj, qq[u].w)) continue;
+            const int32_t hj = hash[jj[u]];
+            bool in = false;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) in |= (hm[m] == hj);
+            if (in) stage_pair<kStage>(i, jj[u], s_pairs, &s_np, pairs, cap, npairs);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int ns = min(s_np, kStage);
+  if (threadIdx.x == 0) s_base = ns > 0 ? atomicAdd(npairs, (unsigned long long)ns) : 0ull;
+  __syncthreads();
+  const unsigned long long base = s_base;
+  for (int u = threadIdx.x; u < ns; u += 256) {
+    const unsigned long long w = base + (unsigned long long)u;
+    if (w < (unsigned long long)cap) reinterpret_cast<int2 *>(pairs)[w] = s_pairs[u];
+  }
+}
+
+// pairs of the large particles: blockIdx.y picks large particle L, threads run over all j; a
+// pair of two large particles is taken by the one with the smaller index
+__global__ __launch_bounds__(256) void k_bp_touch_large(const lbdem_dem_params P, const double *pos,
+                                                        const double *rad, const int32_t *hash, const int32_t *large,
+                                                        int32_t *pairs, int64_t cap, unsigned long long *npairs) {
+  constexpr int kStage = pair_stage_size<8>();
+  __shared__ int2 s_pairs[kStage];
+  __shared__ int s_np;
+  __shared__ unsigned long long s_base;
+  if (threadIdx.x == 0) s_np = 0;
+  __syncthreads();
+  const int L = large[blockIdx.y];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < P.n && j != L && !(rad[j] > P.large_r && j < L)) {
+    if (touching(P, pos + 3 * L, rad[L], pos + 3 * j, rad[j])) {
+      const int a = min(L, j), b = max(L, j);
+      if (probed_by(P, pos + 3 * a, hash[b])) stage_pair<kStage>(a, b, s_pairs, &s_np, pairs, cap, npairs);
+    }
+  }
+  __syncthreads();
+  const int ns = min(s_np, kStage);
+  if (threadIdx.x == 0) s_base = ns > 0 ? atomicAdd(npairs, (unsigned long long)ns) : 0ull;
+  __syncthreads();
+  const unsigned long long base = s_base;
+  for (int u = threadIdx.x; u < ns; u += 256) {
+    const unsigned long long w = base + (unsigned long long)u;
+    if (w < (unsigned long long)cap) reinterpret_cast<int2 *>(pairs)[w] = s_pairs[u];
+  }
+}
+
 // The history slots of candidate pairs that no longer touch are released, as resolve_existing
 // does for every candidate with gap >= 0 (demsolver.py:638-668); contacts whose partner left
 // i's probed buckets keep their slot, as in the reference.  Runs before the resolve kernels.
@@ -1420,6 +1537,29 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
   const unsigned nb = (unsigned)((n + 255) / 256);
   const int ntiles = (P->T + 1023) / 1024;
   hipLaunchKernelGGL(k_bp_count, dim3(nb), dim3(256), 0, s, *P, pos, w->hash, w->count, force, moment);
+  if (rad_touch && P->fine_cell > 0.0) {
+    // the main grid's hashes stay (the release kernel and the candidate filter read them); its
+    // counts are cleared and the second grid is built in the same table
+    hipLaunchKernelGGL(k_bp_reset, dim3(nb), dim3(256), 0, s, *P, w->hash, w->count, w->current);
+    lbdem_dem_params Pf = *P;
+    Pf.cell = P->fine_cell;
+    hipLaunchKernelGGL(k_bp_count, dim3(nb), dim3(256), 0, s, Pf, pos, w->fhash, w->count, nullptr, nullptr);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, w->count, w->offset, w->tile_sum, P->T);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, ntiles);
+    hipLaunchKernelGGL(k_scan_add, dim3((P->T + 255) / 256), dim3(256), 0, s, w->offset, w->tile_sum, P->T);
+    hipLaunchKernelGGL(k_bp_scatter, dim3(nb), dim3(256), 0, s, Pf, w->fhash, w->offset, w->current, w->pid);
+    hipLaunchKernelGGL(k_bp_sort_state, dim3(nb), dim3(256), 0, s, Pf, pos, rad_touch, w->pid, w->spos);
+    const unsigned split = (unsigned)std::min(16, std::max(1, (int)((2048 + nb - 1) / nb)));
+    hipLaunchKernelGGL(k_bp_probe_touch_fine, dim3(nb, std::min(split, 27u)), dim3(256), 0, s, *P, Pf, w->count,
+                       w->offset, w->pid, reinterpret_cast<const double4 *>(w->spos), w->hash, w->pairs,
+                       w->pair_cap, w->npairs);
+    if (P->nlarge > 0)
+      hipLaunchKernelGGL(k_bp_touch_large, dim3(nb, (unsigned)P->nlarge), dim3(256), 0, s, *P, pos, rad_touch,
+                         w->hash, w->large, w->pairs, w->pair_cap, w->npairs);
+    hipLaunchKernelGGL(k_bp_reset, dim3(nb), dim3(256), 0, s, Pf, w->fhash, w->count, w->current);
+    LBDEM_CHECK_LAUNCH(who);
+    return LBDEM_OK;
+  }
   hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, w->count, w->offset, w->tile_sum, P->T);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, ntiles);
   hipLaunchKernelGGL(k_scan_add, dim3((P->T + 255) / 256), dim3(256), 0, s, w->offset, w->tile_sum, P->T);
@@ -1501,6 +1641,14 @@ int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbd
   }
   LBDEM_REQUIRE(w->need_new && w->pairs && w->pair_cap > 0, "lbdem_dem_run: need_new and pairs are required");
   LBDEM_REQUIRE(((uintptr_t)w->spos & 31u) == 0, "lbdem_dem_run: spos must be 32-byte aligned");
+  if (P->fine_cell > 0.0) {
+    LBDEM_REQUIRE(!P->probe27 && !P->periodic[0] && !P->periodic[1] && !P->periodic[2],
+                  "lbdem_dem_run: fine_cell needs the 8-cell probe and no periodic axis");
+    LBDEM_REQUIRE(P->fine_cell > 2.0 * P->large_r && P->large_r > 0.0,
+                  "lbdem_dem_run: fine_cell must exceed 2 large_r > 0");
+    LBDEM_REQUIRE(w->fhash && w->spos && P->nlarge >= 0 && P->nlarge <= 65535 && (P->nlarge == 0 || w->large),
+                  "lbdem_dem_run: fine_cell needs fhash, spos and the large-particle list (at most 65535)");
+  }
   const unsigned nb = (unsigned)((n + 255) / 256);
   const unsigned pb = 2048;  // grid-stride over the candidate list
   for (int sub = 0; sub < nsub; ++sub) {

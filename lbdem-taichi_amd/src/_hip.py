@@ -28,7 +28,7 @@ class Lattice(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 
 class StepArgs(ctypes.Structure):

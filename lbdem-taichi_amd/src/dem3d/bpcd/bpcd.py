@@ -60,7 +60,7 @@ class BPCD:
             z = lambda m, dt=torch.int32: torch.zeros(max(int(m), 1), dtype=dt, device=device)  # noqa: E731
             t = dict(count=z(T), offset=z(T), current=z(T), tile_sum=z((T + 1023) // 1024), hash=z(n), pid=z(n),
                      pairs=z(2 * pair_cap), npairs=z(1, torch.int64), need_new=z(pair_cap),
-                     spos=z(4 * n, torch.float64))
+                     spos=z(4 * n, torch.float64), fhash=z(n))
             w = _abi.DemWork()
             for k, v in t.items():
                 setattr(w, k, v.data_ptr())

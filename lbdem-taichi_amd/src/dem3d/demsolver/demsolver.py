@@ -374,7 +374,33 @@ class DEMSolver:
                 P.wall_n[j][a] = float(self.wf[j]["normal"][a])
             P.wall_d[j] = float(self.wf[j]["distance"])
             P.wall_type[j] = int(self.wf[j]["boundaryType"])
+        self._fine_grid(P)
         return P
+
+    # at most this many particles are tested against all others on the second-grid search
+    MAX_LARGE = 64
+
+    def _fine_grid(self, P):
+        """Second-grid touching search of the grid-wide path (lbdem_dem_params.fine_cell): used when
+        all but at most MAX_LARGE particles have radii at most 1/8 of the broad-phase cell (a
+        strongly polydisperse packing, whose 4 r_max cells hold most particles in a few buckets).
+        The pairs are the same; only the search changes."""
+        P.fine_cell, P.large_r, P.nlarge = 0.0, 0.0, 0
+        self._large = None
+        general = self.path == 2 or (self.path == 0 and self.n > SMALL_PATH_MAX)
+        if not general or P.probe27 or any(P.periodic[a] for a in range(3)):
+            return
+        r = self.gf.radius.to_numpy()
+        small = r[r <= P.cell / 8.0]
+        if small.size == 0:
+            return
+        large = np.nonzero(r > small.max())[0]
+        if large.size > self.MAX_LARGE:
+            return
+        P.large_r = float(small.max())
+        P.fine_cell = 2.0 * P.large_r * (1.0 + 1e-6)
+        P.nlarge = int(large.size)
+        self._large = torch.as_tensor(large.astype(np.int32), device=self.device)
 
     def _state(self):
         g = self.gf
@@ -400,6 +426,8 @@ class DEMSolver:
         if nsub <= 0 or self.n == 0:
             return
         cached = getattr(self, "_cached", None)
+        if cached is not None and self.gf.radius.version != self._radius_version:
+            cached = self._cached = None  # radii written: the second-grid parameters follow them
         if cached is None:
             P = self._params()
             S = self._state()
@@ -408,6 +436,7 @@ class DEMSolver:
             t, w = self.bpcd.workspace(self.n, self._pair_cap, self.device)
             w.verlet = self._verlet.data_ptr()
             w.verlet_ref = self._verlet_ref.data_ptr()
+            w.large = self._large.data_ptr() if self._large is not None else None
             cached = self._cached = (P, S, w, ctypes.byref(P), ctypes.byref(S), ctypes.byref(w), t)
             self._verlet[:1].zero_()
         else:
