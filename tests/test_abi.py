@@ -117,3 +117,39 @@ def test_reduce_static_rejects_a_lattice_beyond_its_scratch():
     rc = L.lbdem_lbm_reduce_static(ctypes.byref(lat), ctypes.byref(args), ctypes.cast(out, ctypes.c_void_p), None)
     assert rc == -1
     assert b"exceed the static scratch" in L.lbdem_last_error()
+
+
+def test_dem_run_rejects_a_second_grid_it_cannot_use():
+    """The second-grid touching search (fine_cell > 0) is checked on the host, before any HIP
+    call: it needs the 8-cell probe, no periodic axis, fine_cell > 2 large_r and its buffers."""
+    import src._hip as H
+    from src.dem3d import _abi
+    L = H.lib()
+    buf = (ctypes.c_double * 64)()
+    addr = ctypes.addressof(buf)
+    base = addr + (-addr) % 64
+
+    def call(**kw):
+        P = _abi.DemParams()
+        P.n, P.K, P.T, P.cell, P.dt = 5000, 8, 1 << 20, 1.0, 1e-3
+        P.fine_cell, P.large_r, P.nlarge = 0.3, 0.1, 1
+        S = _abi.DemState()
+        S.status = base
+        w = _abi.DemWork()
+        w.pairs = w.need_new = w.npairs = w.fhash = w.large = w.spos = base
+        w.count = w.offset = w.current = w.hash = w.pid = w.tile_sum = base
+        w.pair_cap = 16
+        for k, v in kw.items():
+            if k in ("fhash", "large", "spos"):
+                setattr(w, k, v)
+            elif k == "periodic":
+                P.periodic[0] = v
+            else:
+                setattr(P, k, v)
+        return L.lbdem_dem_run(ctypes.byref(P), ctypes.byref(S), ctypes.byref(w), 1, 2, None)
+
+    for kw, msg in ((dict(probe27=1), b"8-cell probe"), (dict(periodic=1), b"8-cell probe"),
+                    (dict(fine_cell=0.2), b"exceed 2 large_r"), (dict(fhash=None), b"large-particle list"),
+                    (dict(large=None), b"large-particle list"), (dict(nlarge=70000), b"large-particle list")):
+        assert call(**kw) == -1, kw
+        assert msg in L.lbdem_last_error(), (kw, L.lbdem_last_error())

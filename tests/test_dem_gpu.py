@@ -89,14 +89,17 @@ def test_dem_trajectories_match_oracle(model, path):
     assert gpu.status()["slot_overflow"] == 0
 
 
+@pytest.mark.parametrize("search", ["second_grid", "tiled"])
 @pytest.mark.parametrize("offset", [0.0, 1000.0])
-def test_dem_coarse_grid_polydisperse_matches_oracle(offset):
+def test_dem_coarse_grid_polydisperse_matches_oracle(offset, search):
     """Two grains of R 0.13 / 0.11 among ~2,900 of R <= 0.02: the broad-phase cell (4 r_max)
-    makes a 2 x 2 x 2 grid, so each particle has ~1,000 candidates and the multi-kernel probe
-    stages whole buckets in LDS.  Trajectories and contact sets follow the oracle.  With the
-    domain at [1000, 1001]^3 the staged probe's FP32 pre-test works on coordinates rounded by up
-    to 3e-5, the size of the contact overlaps, so only its rounding slack (2^-20 of the
-    coordinates' magnitude) keeps every touching pair."""
+    makes a 2 x 2 x 2 grid, so each particle has ~1,000 candidates.  'second_grid': the
+    grid-wide path's default here, the small grains on a 0.04-wide grid and the two large ones
+    against all; 'tiled' (second grid switched off): the probe stages whole buckets in LDS.
+    Trajectories and contact sets follow the oracle.  With the domain at [1000, 1001]^3 the
+    staged probe's FP32 pre-test works on coordinates rounded by up to 3e-5, the size of the
+    contact overlaps, so only its rounding slack (2^-20 of the coordinates' magnitude) keeps
+    every touching pair."""
     rng = np.random.default_rng(3)
     box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     big = np.array([[0.3, 0.35, 0.4], [0.68, 0.62, 0.55]])
@@ -111,10 +114,14 @@ def test_dem_coarse_grid_polydisperse_matches_oracle(offset):
     box = tuple(b + offset for b in box)
     cfg = make_config(model="hertz", box=box, dt=5e-5)
     gpu = gpu_solver(cfg, rad, mass, pos, vel, path=2)
+    if search == "tiled":
+        gpu.MAX_LARGE = 1
     orc = oracle_solver(cfg, rad, mass, pos, vel)
     for _ in range(4):
         gpu.run_substeps(50)
         orc.step(50)
+    P = gpu._cached[0]
+    assert (P.fine_cell > 0.0 and P.nlarge == 2) if search == "second_grid" else P.fine_cell == 0.0
     assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
     assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
     K = gpu.K
