@@ -486,9 +486,30 @@ class BasicLattice3D:
         return _FeqView(self)
 
     @property
+    def rate(self):
+        """Strain-rate magnitude of the last collision (the `rate` field, lbm_solver3d.py:133,
+        461-498), recovered from that collision's relaxation frequency, which the lagged
+        Smagorinsky keeps: omega = 1 / (1/omega0 + Cs^2 rate / cs^2) (:500-513)."""
+        return _RateView(self)
+
+    @property
     def fpc(self):
         raise AttributeError("post-collision populations are not materialised: the collide-stream "
                              "kernel writes them straight into the streamed slots")
+
+
+class _RateView:
+    def __init__(self, lattice):
+        self._lat = lattice
+
+    def to_numpy(self):
+        L = self._lat
+        cs = float(type(L).SmagorinskyConstant)
+        if L._lag is None or cs == 0.0:
+            raise AttributeError("rate is kept only by the lagged Smagorinsky collision (smagorinsky='lagged', "
+                                 "SmagorinskyConstant > 0): the other collisions store no relaxation frequency")
+        om = L.omega.to_numpy().astype(np.float64)
+        return (1.0 / om - 1.0 / L.omega0) * L.cssq / (cs * cs)
 
 
 class _FeqView:

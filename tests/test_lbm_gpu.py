@@ -35,6 +35,31 @@ def test_cavity_lagged_smagorinsky_f32(smag017, steps):
     assert rel_err(f[fluid], orc.f[fluid]) < 1e-5
 
 
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-9), ("f32", 2e-2)])
+def test_strain_rate_field_matches_the_reference_formula(smag017, storage, tol):
+    """`rate` after step k is the reference's strain rate of that collision,
+    omega_{k-1} / (2 cs^2 rho) * |Pi_neq(f - feq_{k-1})| (lbm_solver3d.py:461-498), evaluated here
+    in numpy on the oracle's state after k - 1 steps.  FP32 storage loses digits in
+    1/omega - 1/omega0, hence its looser bound (relative to the largest rate)."""
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    orc = oracle_cavity(16)
+    gpu = gpu_from_oracle(BasicLattice3D, orc, storage=storage)
+    orc.initialize()
+    gpu.initialize()
+    orc.step(60)
+    c = np.array(BasicLattice3D.c, dtype=np.float64)
+    fneq = orc.f - orc.feq
+    P = np.einsum("...q,qa,qb->...ab", fneq, c, c)
+    Pi = np.sqrt(2.0 * (P ** 2).sum(axis=(-1, -2)))
+    rate = orc.omega / (2.0 / 3.0 * orc.rho) * Pi
+    orc.step(1)
+    gpu.step(61)
+    fluid = orc.CT == O.FLUID
+    assert np.abs(orc.rate[fluid] - rate[fluid]).max() < 1e-12 * np.abs(rate[fluid]).max()
+    got = gpu.rate.to_numpy()
+    assert np.abs(got[fluid] - rate[fluid]).max() < tol * np.abs(rate[fluid]).max()
+
+
 def test_cavity_f64_storage_matches_oracle_tightly(smag017):
     from src.lbm3d.lbm_solver3d import BasicLattice3D
     orc = oracle_cavity(16)
