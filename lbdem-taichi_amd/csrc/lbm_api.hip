@@ -272,14 +272,18 @@ int lbdem_lbm_step(const lbdem_lattice *lat, const lbdem_step_args *a, void *str
                 "lbdem_lbm_step: VANS needs volfrac, hforce, volfrac_prev, hforce_prev, nu_lu > 0");
   // row-tiled variants: whole rows per workgroup, LDS staging of the z shift (<= 64 KiB);
   // kernel 2 / 3 / 4: one / two / four cells per thread.  Automatic choice (kernel 0), the
-  // fastest measured on MI355X (256^3 TGV): one cell per thread for FP32 (4-byte accesses), two
-  // for FP64 (16 bytes), four for 16-bit storage (8 bytes; two where nz is not a multiple of 4).
+  // fastest measured on MI355X for each parity (256^3 TGV; all variants give identical bits):
+  // one cell per thread for FP32 and FP64; for 16-bit storage two cells in the even step and
+  // four in the odd one (BF16 235 / 260 us against 245 / 260 with four in both), and with the
+  // lagged Smagorinsky one and two (340 / 379 us against 445 / 441: with the lag state the
+  // four-cell variant holds 138 VGPRs, three wavefronts per SIMD; one cell 57, eight).
+  // Narrower where the row does not fit.
   int kern = lat->kernel;
   LBDEM_REQUIRE(kern >= 0 && kern <= 4, "lbdem_lbm_step: bad kernel variant %d", kern);
   if (kern == 0) {
-    if (lat->storage == LBDEM_STORE_F32) kern = 2;
-    else if (lat->storage == LBDEM_STORE_F64) kern = 3;
-    else kern = lat->nz % 4 == 0 ? 4 : 3;
+    if (lat->storage == LBDEM_STORE_F32 || lat->storage == LBDEM_STORE_F64) kern = 2;
+    else if (coll == LBDEM_COLL_SMAG_LAGGED) kern = p.t_odd ? 3 : 2;
+    else kern = p.t_odd ? (lat->nz % 4 == 0 ? 4 : 3) : 3;
   }
   const int tsize = lat->storage == LBDEM_STORE_F64 ? 8 : 4;
   auto fits = [&](int v) {
