@@ -573,7 +573,8 @@ def precision_study(n=256, steps=20000, every=500, checkpoints=(1000, 5000, 1000
             "smagorinsky": {"Cs": 0.1, "mode": smag}, "results": out}
 
 
-def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=10, relax=4000, K=64, tau=0.503):
+def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=10, relax=4000, K=64, tau=0.503,
+                     cpu=True):
     """Unresolved EqIBM LBM-DEM at its full single-GPU shape: 128^3 lattice (walls on all faces),
     200,000 spheres with log-normal diameters (mean 0.5 lu, sigma_log 0.2, clipped below 1 lu) in
     a 64 x 64 x 9 lu bed on the floor (solid fraction 0.4, porosity 0.6), density ratio 2.5,
@@ -593,7 +594,10 @@ def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=
     timed step is: DEM substeps (general path, gravity -1e-5 lu), collide-stream with the
     Tenneti weights, update_coupling (spreading, finalize, drag).  Reports per-phase times (HIP
     events), the step's MLUPS, the DEM status counters of the timed steps (dem.check()) and
-    whether every grain's position and velocity is finite at the end (the study raises if not)."""
+    whether every grain's position and velocity is finite at the end (the study raises if not).
+    With `cpu`, the oracle (the C restatement of the reference's algorithms) runs a bounded sample
+    of the same step on the host from the final state: 2 DEM substeps (serial) and one lattice
+    step with the EqIBM coupling (collide and stream on all host threads)."""
     import numpy as np
     import torch
     from src.dem3d.demconfig import DEMSolverConfig, DomainBounds, LinearContactConfig
@@ -677,7 +681,34 @@ def unresolved_study(seed=2027, n=200_000, N=128, steps=20, warmup=3, subcycles=
     finite = bool(np.isfinite(p_end).all() and np.isfinite(v_end).all())
     if not finite:
         raise RuntimeError("unresolved study: grain positions or velocities are not finite after the timed steps")
-    return {"study": "unresolved", "lattice": N, "particles": n, "steps": steps, "subcycles": subcycles, "tau": tau,
+    out_cpu = {}
+    if cpu:
+        from oracle import oracle as O
+        odem = O.OracleDEM(dem.config, rad, mass, p_end, v_end, K=K, cell=float(d.max()), probe27=True)
+        t0 = time.perf_counter()
+        odem.step(2)
+        dem_s = (time.perf_counter() - t0) / 2
+        orc = O.OracleLattice(N, N, N, 1.0 / tau, kind="eqibm", cs=EqIMBlattice3D.SmagorinskyConstant)
+        orc.CT[...] = walls_ct(N, N, N)
+        orc.initialize()
+        g = np.column_stack([p_end, v_end, rad])
+        O.ibm_spread(orc, g, [0.0, 0.0, 0.0], 1.0, 1.0)
+        O.ibm_weight(orc, 1.0, 1.0, 1.0)
+        t0 = time.perf_counter()
+        orc.collide()
+        orc.stream()
+        O.ibm_spread(orc, g, [0.0, 0.0, 0.0], 1.0, 1.0)
+        O.ibm_weight(orc, 1.0, 1.0, 1.0)
+        O.ibm_drag(orc, g, [0.0, 0.0, 0.0], 1.0, 1.0, 1.0)
+        lat_s = time.perf_counter() - t0
+        step_s = lat_s + subcycles * dem_s
+        out_cpu = {"cpu_sample": "oracle: 2 DEM substeps (serial C) and one EqIBM lattice step (collide/stream on "
+                                 f"{os.environ.get('OMP_NUM_THREADS', os.cpu_count())} OpenMP threads, coupling serial)",
+                   "cpu_dem_substep_s": round(dem_s, 3), "cpu_lattice_step_s": round(lat_s, 3),
+                   "cpu_step_s": round(step_s, 3), "gpu_over_cpu": round(step_s / (ms * 1e-3), 1)}
+        del orc, odem
+    return {**out_cpu, "study": "unresolved", "lattice": N, "particles": n, "steps": steps, "subcycles": subcycles,
+            "tau": tau,
             "bed": "64 x 64 x 9 lu, solid fraction 0.40", "contacts": "linear, k_n = k_t = 1e-2, damping 0.3",
             "relaxation_substeps": relax, "grains_finite": finite, "grain_speed_max": float(np.abs(v_end).max()),
             "bed_top": float(p_end[:, 2].max()),
