@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBDEM_ABI_VERSION 8
+#define LBDEM_ABI_VERSION 9
 
 /* return codes */
 #define LBDEM_OK 0
@@ -69,6 +69,10 @@ extern "C" {
 #define LBDEM_FLAG_FLUID 0x10  /* rho,u are refreshed after streaming (CellType.FLUID) */
 #define LBDEM_FLAG_INLET 0x20  /* with MOVING: Ladd velocity inlet (CellType.VEL_INLET_LADD); its
                                   Ladd term uses this cell's density (step_args.rho) */
+#define LBDEM_FLAG_ROW_IDLE 0x40 /* on the first cell (k = 0) of a row (fixed i, j) that has no ACTIVE
+                                    cell: the row-tiled step leaves such rows alone (their step
+                                    would store every slot unchanged).  Set and cleared by
+                                    lbdem_lbm_prepare_flags; derived state, not an input */
 
 /* wet-node boundary cells (lbdem_lbm_boundaries): kind = type | face << 4 */
 #define LBDEM_BC_VEL_ZOUHE 1   /* CellType.VEL_ZOUHE: Zou-He velocity (vel given, rho computed) */

@@ -207,6 +207,19 @@ __global__ __launch_bounds__(256) void k_prepare_flags(const KParams p, uint8_t 
   nmask[c] = m;
 }
 
+// LBDEM_FLAG_ROW_IDLE on the first cell of each row with no active cell (one thread per row;
+// runs after k_prepare_flags, which keeps the bit as it is)
+__global__ __launch_bounds__(256) void k_mark_idle_rows(const KParams p, uint8_t *flags) {
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if (row >= p.nx * p.ny) return;
+  const size_t base = (size_t)row * p.nz;
+  bool any = false;
+  for (int k = 0; k < p.nz && !any; ++k) any = flags[base + k] & LBDEM_FLAG_ACTIVE;
+  const uint8_t fl = flags[base];
+  const uint8_t idle = any ? 0 : LBDEM_FLAG_ROW_IDLE;
+  if ((fl & LBDEM_FLAG_ROW_IDLE) != idle) flags[base] = (uint8_t)((fl & ~LBDEM_FLAG_ROW_IDLE) | idle);
+}
+
 }  // namespace lbdem
 
 using namespace lbdem;
@@ -227,6 +240,8 @@ int lbdem_lbm_prepare_flags(const lbdem_lattice *lat, uint8_t *flags, uint32_t *
   p.ncell = (int)n;
   hipLaunchKernelGGL(k_prepare_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), p, flags,
                      nmask);
+  hipLaunchKernelGGL(k_mark_idle_rows, dim3((unsigned)((lat->nx * lat->ny + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), p, flags);
   LBDEM_CHECK_LAUNCH("lbdem_lbm_prepare_flags");
   return LBDEM_OK;
 }

@@ -1010,6 +1010,10 @@ struct Vec {
 
 template <typename S, int COLL, int COUP, bool ODD, int V>
 __global__ __launch_bounds__(pencil_max_threads(V)) void k_step_pencil(const KParams p) {
+  // a row without active cells would store every slot unchanged: walls store what they loaded
+  // (even step: their own slots; odd step: their active targets' slots, and anything into the
+  // slots of wall targets, which nothing reads)
+  if (p.flags[(size_t)blockIdx.x * p.nz] & LBDEM_FLAG_ROW_IDLE) return;
   using T = typename Store<S>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   T *buf = reinterpret_cast<T *>(smem);  // [kNZ][ncz]

@@ -16,6 +16,7 @@ STORE_DTYPE = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat1
 COLL_BGK, COLL_SMAG_LAGGED, COLL_SMAG_FIXED = 0, 1, 2
 COUPLE_NONE, COUPLE_PSC, COUPLE_EQIBM, COUPLE_VANS, COUPLE_HYBRID = 0, 1, 2, 3, 4
 FLAG_ACTIVE, FLAG_NEAR, FLAG_WALL, FLAG_MOVING, FLAG_FLUID, FLAG_INLET = 0x01, 0x02, 0x04, 0x08, 0x10, 0x20
+FLAG_ROW_IDLE = 0x40
 BC_VEL_ZOUHE, BC_PRE_ZOUHE, BC_EXIT, BC_LADD_INLET = 1, 2, 3, 4
 
 _vp = ctypes.c_void_p
@@ -28,7 +29,7 @@ class Lattice(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 
 class StepArgs(ctypes.Structure):

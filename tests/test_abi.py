@@ -74,7 +74,7 @@ def test_header_constants_match_python_binding():
     text = open(HEADER).read()
     consts = {m.group(1): int(m.group(2), 0) for m in re.finditer(r"#define\s+(LBDEM_[A-Z0-9_]+)\s+(0x[0-9a-fA-F]+|\d+)",
                                                                       text)}
-    flags = ("ACTIVE", "NEAR", "WALL", "MOVING", "FLUID", "INLET")
+    flags = ("ACTIVE", "NEAR", "WALL", "MOVING", "FLUID", "INLET", "ROW_IDLE")
     for name in flags:
         assert consts[f"LBDEM_FLAG_{name}"] == getattr(H, f"FLAG_{name}"), name
     assert len({consts[f"LBDEM_FLAG_{n}"] for n in flags}) == len(flags)

@@ -225,6 +225,34 @@ def test_row_tiled_kernel_equals_per_cell_kernel(smag017, storage, steps):
         assert np.array_equal(out[0], out[m]), f"kernel {kernels[m]}"
 
 
+def test_idle_rows_are_marked_and_skipped(smag017):
+    """lbdem_lbm_prepare_flags marks the first cell of every row without an active cell
+    (LBDEM_FLAG_ROW_IDLE) and nothing else; the row-tiled kernel, which skips those rows, stays
+    bit-identical to the per-cell kernel on a cavity with a solid interior slab and a row whose
+    only active cell is at its far end."""
+    from src import _hip
+    from src.lbm3d.lbm_solver3d import BasicLattice3D
+    orc = oracle_cavity(16)
+    orc.CT[5:7, 3:9, 1:-1] = O.OBSTACLE          # rows (5..6, 3..8) entirely solid
+    orc.CT[9, 9, 1:-2] = O.OBSTACLE               # row (9, 9): one fluid cell left, at k = 14
+    rng = np.random.default_rng(5)
+    fluid = orc.CT == O.FLUID
+    orc.vel[fluid] = rng.uniform(-0.03, 0.03, size=orc.vel[fluid].shape).astype(np.float32)
+    out = []
+    for kernel in (1, 2):
+        L = gpu_from_oracle(BasicLattice3D, orc)
+        L.kernel = kernel
+        L.initialize()
+        L.step(41)
+        out.append(L.f.to_numpy())
+        fl = L._flags.cpu().numpy().reshape(orc.CT.shape)
+    assert np.array_equal(out[0], out[1])
+    idle = ~(fl & _hip.FLAG_ACTIVE).astype(bool).any(axis=2)
+    assert idle[5, 3] and not idle[9, 9] and idle[0].all()
+    assert np.array_equal((fl[:, :, 0] & _hip.FLAG_ROW_IDLE) != 0, idle)
+    assert not (fl[:, :, 1:] & _hip.FLAG_ROW_IDLE).any()
+
+
 @pytest.mark.parametrize("shape", [(4, 3, 600), (3, 2, 400), (5, 4, 131), (3, 3, 64)])
 @pytest.mark.parametrize("storage", ["f32", "bf16"])
 def test_kernel_variants_long_rows(smag017, shape, storage):
