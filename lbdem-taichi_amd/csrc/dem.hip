@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_bp_count(const lbdem_dem_params P, cons
     h = hash3(c[0], c[1], c[2], P.T);
     hash[i] = h;
   }
-  agg_add_one(count, h, active);
+  if (count) agg_add_one(count, h, active);  // NULL: hashes only
   if (!active) return;
   if (force) {
 #pragma unroll
@@ -1536,11 +1536,12 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
   const int n = P->n;
   const unsigned nb = (unsigned)((n + 255) / 256);
   const int ntiles = (P->T + 1023) / 1024;
-  hipLaunchKernelGGL(k_bp_count, dim3(nb), dim3(256), 0, s, *P, pos, w->hash, w->count, force, moment);
-  if (rad_touch && P->fine_cell > 0.0) {
-    // the main grid's hashes stay (the release kernel and the candidate filter read them); its
-    // counts are cleared and the second grid is built in the same table
-    hipLaunchKernelGGL(k_bp_reset, dim3(nb), dim3(256), 0, s, *P, w->hash, w->count, w->current);
+  const bool fine = rad_touch && P->fine_cell > 0.0;
+  // with the second grid the main grid needs its hashes only (the release kernel and the
+  // candidate filter read them); the second grid is built in the same table
+  hipLaunchKernelGGL(k_bp_count, dim3(nb), dim3(256), 0, s, *P, pos, w->hash, fine ? nullptr : w->count, force,
+                     moment);
+  if (fine) {
     lbdem_dem_params Pf = *P;
     Pf.cell = P->fine_cell;
     hipLaunchKernelGGL(k_bp_count, dim3(nb), dim3(256), 0, s, Pf, pos, w->fhash, w->count, nullptr, nullptr);
