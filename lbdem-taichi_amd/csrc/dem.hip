@@ -706,10 +706,10 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
 
 // The touching probe of the 27-cell rule over particles in bucket order (spos): lane t takes
 // the t-th particle of the bucket order, so a wavefront's particles share most of their buckets
-// and read their candidates from neighbouring 32-byte records.  The walk is latency-bound (a
-// bucket's header, then its entries), so blockIdx.y splits each particle's 27 buckets, not
-// their entries, over gridDim.y workgroups, and a bucket's entries are loaded four at a time:
-// one chain of two dependent loads per bucket and a few buckets per lane.
+// and read their candidates from neighbouring 32-byte records.  The walk is latency-bound, so
+// blockIdx.y (three workgroups) splits each particle's 27 buckets into the three z slabs of 9,
+// not their entries; a lane loads its slab's 9 bucket headers together, then walks the 9
+// buckets' entries as one range, four loads at a time (eight measured no faster).
 __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_params P, const int32_t *count,
                                                                const int32_t *offset, const int32_t *pid,
                                                                const double4 *spos, int32_t *pairs, int64_t cap,
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_p
   if (threadIdx.x == 0) s_np = 0;
   __syncthreads();
   const int t0 = blockIdx.x * 256 + threadIdx.x;
-  const int split = gridDim.y, part = blockIdx.y;
+  const int part = blockIdx.y;  // gridDim.y == 3
   if (t0 < P.n) {
     const int i = pid[t0];
     const double4 q = spos[t0];
@@ -730,27 +730,43 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_p
     int32_t c[3], h[27];
     cell_of(P, xi, c);
     probe_cells<27>(P, xi, c, h);
+    // this workgroup's 9 buckets, k = 3 m + part (the cells at z offset part - 1): all headers
+    // first, then the entries of the 9 buckets as one range, four loads at a time
+    int32_t beg[9], pre[9];
+    int total = 0;
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-      if (k % split != part || h[k] < 0) continue;
-      const int32_t b = offset[h[k]], e = b + count[h[k]];
-      for (int idx = b; idx < e; idx += 4) {
-        int32_t jj[4];
-        double4 qq[4];
+    for (int m = 0; m < 9; ++m) {
+      const int32_t hk = part == 0 ? h[3 * m] : (part == 1 ? h[3 * m + 1] : h[3 * m + 2]);
+      int32_t b = 0, cn = 0;
+      if (hk >= 0) {
+        b = offset[hk];
+        cn = count[hk];
+      }
+      pre[m] = total;
+      beg[m] = b - total;  // entry e of the range, pre[m] <= e < pre[m] + cn, is at beg[m] + e
+      total += cn;
+    }
+    for (int e0 = 0; e0 < total; e0 += 4) {
+      int32_t jj[4];
+      double4 qq[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          jj[u] = -1;
-          if (idx + u < e) {
-            jj[u] = pid[idx + u];
-            qq[u] = spos[idx + u];
-          }
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + u;
+        jj[u] = -1;
+        if (e < total) {
+          int32_t b = beg[0];  // the last bucket starting at or before e holds it
+#pragma unroll
+          for (int m = 1; m < 9; ++m)
+            if (e >= pre[m]) b = beg[m];
+          jj[u] = pid[b + e];
+          qq[u] = spos[b + e];
         }
+      }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (jj[u] <= i) continue;
-          const double xj[3] = {qq[u].x, qq[u].y, qq[u].z};
-          if (touching(P, xi, ri, xj, qq[u].w)) stage_pair<kStage>(i, jj[u], s_pairs, &s_np, pairs, cap, npairs);
-        }
+      for (int u = 0; u < 4; ++u) {
+        if (jj[u] <= i) continue;
+        const double xj[3] = {qq[u].x, qq[u].y, qq[u].z};
+        if (touching(P, xi, ri, xj, qq[u].w)) stage_pair<kStage>(i, jj[u], s_pairs, &s_np, pairs, cap, npairs);
       }
     }
   }
@@ -1570,8 +1586,10 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
     // there are fewer than 2048 workgroups (more waves in flight; also measured faster on the
     // sparse 200,000-particle bed, 0.72 against 0.84 ms with no split)
     const unsigned split = (unsigned)std::min(16, std::max(1, (int)((2048 + nb - 1) / nb)));
-    // the 27-cell probe (fine grids) walks particles in bucket order (200,000-grain bed: 0.20
-    // ms per substep with a per-entry split, 0.54 ms in particle order; a walk of each
+    // the 27-cell probe (fine grids) walks particles in bucket order (200,000-grain bed: 0.102
+    // ms per substep with z slabs and the headers first, 0.116 ms with the buckets split k % 3
+    // and walked one after another, 0.20 ms with a per-entry split, 0.54 ms in particle order;
+    // 9 and 27 workgroups per particle measured slower than 3; a walk of each
     // wavefront's box of cells together, one candidate per lane and readlane, measured 0.47 ms:
     // waves at jumps of the Morton order get boxes of hundreds of cells and set the launch's
     // tail).  The 8-cell probe serves coarse grids, whose workgroups mostly take its LDS-tiled
@@ -1579,7 +1597,7 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
     const double4 *sp = P->probe27 ? reinterpret_cast<const double4 *>(w->spos) : nullptr;
     if (sp) {
       hipLaunchKernelGGL(k_bp_sort_state, dim3(nb), dim3(256), 0, s, *P, pos, rad_touch, w->pid, w->spos);
-      hipLaunchKernelGGL(k_bp_probe_touch_sorted, dim3(nb, std::min(split, 27u)), dim3(256), 0, s, *P, w->count,
+      hipLaunchKernelGGL(k_bp_probe_touch_sorted, dim3(nb, 3u), dim3(256), 0, s, *P, w->count,
                          w->offset, w->pid, sp, w->pairs, w->pair_cap, w->npairs);
     } else if (P->probe27)
       hipLaunchKernelGGL(k_bp_probe_touch<27>, dim3(nb, split), dim3(256), 0, s, *P, pos, rad_touch, w->count,
