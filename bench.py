@@ -912,6 +912,11 @@ def main():
         ex["c1_cavity64"] = cavity_c1(cpu=not args.no_cpu_baseline)
         ex["c2_coupled_psc"] = coupled_c2(args.seed)
         ex["c5_adjoint"] = adjoint_c5()
+        u = unresolved_study(cpu=not args.no_cpu_baseline)
+        ex["c3_unresolved"] = {k: u[k] for k in ("lattice", "particles", "subcycles", "tau", "ms_per_step", "mlups",
+                                                 "dem_substep_ms", "phase_ms", "grains_finite",
+                                                 "dem_status_timed_steps") if k in u}
+        ex["c3_unresolved"].update({k: u[k] for k in ("cpu_step_s", "gpu_over_cpu", "cpu_sample") if k in u})
         out["extras"] = ex
     print(json.dumps(out), flush=True)
 
