@@ -124,8 +124,9 @@ __global__ __launch_bounds__(256) void k_eq_spread(const IbmArgs a) {
 // cells x 5 fields with global atomics (14 million per step for 200,000 grains, 0.46 ms); in a
 // spatial order without the LDS stage, concurrent wavefronts hit the same cells and it is
 // slower still (0.63 ms).  A workgroup whose union box exceeds kSpreadCells adds directly.
-constexpr int kSpreadGrains = 128;
+constexpr int kSpreadGrains = 128;  // 256 measured slower (0.31 -> 0.41 ms per update_coupling)
 constexpr int kSpreadCells = 1536;
+static_assert(kSpreadGrains <= 256, "the union box is built with one thread per grain of a 256-thread workgroup");
 
 __global__ __launch_bounds__(256) void k_eq_spread_ordered(const IbmArgs a) {
   __shared__ float s_acc[5][kSpreadCells];
