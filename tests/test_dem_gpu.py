@@ -133,6 +133,33 @@ def test_dem_coarse_grid_polydisperse_matches_oracle(offset, search):
     gpu.check()
 
 
+def test_dem_second_grid_without_large_particles_matches_oracle():
+    """A broad-phase cell far above 4 r_max (cell_size = 10 r_max, set by the caller) takes the
+    second-grid search with no large particle: every pair comes from the fine grid and the
+    8-cell candidate filter.  Trajectories and contact sets against the oracle with the same cell."""
+    rng = np.random.default_rng(17)
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    rad, mass, pos, vel = random_packing(2400, 0.018, box, rng, v=1.0, polydisperse=0.3)
+    cell = float(10.0 * rad.max())
+    cfg = make_config(model="hertz", box=box, dt=5e-5)
+    gpu = gpu_solver(cfg, rad, mass, pos, vel, path=2, cell_size=cell)
+    orc = oracle_solver(cfg, rad, mass, pos, vel, cell=cell)
+    for _ in range(4):
+        gpu.run_substeps(50)
+        orc.step(50)
+    P = gpu._cached[0]
+    assert P.fine_cell > 0.0 and P.nlarge == 0
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+    assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
+    K = gpu.K
+    act = gpu._cf_active.cpu().numpy().astype(bool)
+    got = sorted(zip(np.repeat(np.arange(gpu.n), K)[act].tolist(), gpu._cf_j.cpu().numpy()[act].tolist()))
+    oact = orc["cf_active"].astype(bool)
+    ref = sorted(zip(np.repeat(np.arange(orc.n), K)[oact].tolist(), orc["cf_j"][oact].tolist()))
+    assert got == ref and len(ref) > 20
+    gpu.check()
+
+
 def _dense_bed(n, ext, seed):
     """Grains of log-normal diameter (mean 0.5, sigma_log 0.2, below 1) seeded uniformly, with
     overlaps, in a box of extents ext (the unresolved bed's size law)."""
