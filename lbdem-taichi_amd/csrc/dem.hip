@@ -120,55 +120,84 @@ __global__ __launch_bounds__(256) void k_bp_count(const lbdem_dem_params P, cons
   }
 }
 
-// exclusive scan of count[T] -> offset[T]: per-1024 tiles, tile sums, then add
-__global__ __launch_bounds__(256) void k_scan_tiles(const int32_t *in, int32_t *out, int32_t *tile_sum, int n) {
-  __shared__ int32_t sh[256];
-  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-  int32_t v[4], s = 0;
+// exclusive scan of count[T] -> offset[T] in two passes over tiles of kScanTile entries: the
+// tiles' sums, then each tile's scan plus the sums of the tiles before it (read back from L2 by
+// every workgroup; at most 2048 of them for T = 2^22).  Eight entries per lane, as two 16-byte
+// accesses when the arrays allow it (the branch is uniform: a lane's first entry is a multiple of
+// 8 entries into the array).  The three-pass form (tile scans, a one-workgroup scan of the sums,
+// an add pass) took 17 us per substep on the 200,000-grain bed's 2^21-entry table.
+constexpr int kScanTile = 2048;
+
+__device__ __forceinline__ void scan_load8(const int32_t *in, int base, int n, int32_t v[8]) {
+  if (base + 8 <= n && ((uintptr_t)in & 15u) == 0) {
+    const int4 a = *reinterpret_cast<const int4 *>(in + base);
+    const int4 b = *reinterpret_cast<const int4 *>(in + base + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    v[m] = (base + m < n) ? in[base + m] : 0;
-    s += v[m];
+    for (int m = 0; m < 8; ++m) v[m] = (base + m < n) ? in[base + m] : 0;
   }
-  sh[threadIdx.x] = s;
+}
+
+__device__ __forceinline__ int32_t wave_sum(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int32_t *in, int32_t *tile_sum, int n) {
+  __shared__ int32_t sh[4];
+  int32_t v[8];
+  scan_load8(in, blockIdx.x * kScanTile + threadIdx.x * 8, n, v);
+  const int32_t s = wave_sum(v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7]);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {
-    const int32_t t = (threadIdx.x >= (unsigned)o) ? sh[threadIdx.x - o] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int32_t run = sh[threadIdx.x] - s;  // exclusive prefix of this thread
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const int32_t *in, int32_t *out, const int32_t *tile_sum,
+                                                   int n) {
+  __shared__ int32_t sh[8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int32_t c = 0;
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) c += tile_sum[k];
+  c = wave_sum(c);
+  const int base = blockIdx.x * kScanTile + threadIdx.x * 8;
+  int32_t v[8];
+  scan_load8(in, base, n, v);
+  const int32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
+  int32_t x = s;  // inclusive scan of the lanes' sums over the wavefront
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    if (base + m < n) out[base + m] = run;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 0) sh[wave] = c;
+  if (lane == 63) sh[4 + wave] = x;
+  __syncthreads();
+  int32_t run = sh[0] + sh[1] + sh[2] + sh[3] + x - s;
+  for (int w = 0; w < wave; ++w) run += sh[4 + w];
+  int32_t o8[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    o8[m] = run;
     run += v[m];
   }
-  if (threadIdx.x == 255) tile_sum[blockIdx.x] = sh[255];
-}
-__global__ __launch_bounds__(1024) void k_scan_sums(int32_t *tile_sum, int ntiles) {
-  __shared__ int32_t sh[1024];
-  int32_t carry = 0;
-  for (int base = 0; base < ntiles; base += 1024) {
-    const int i = base + threadIdx.x;
-    const int32_t v = (i < ntiles) ? tile_sum[i] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      const int32_t t = (threadIdx.x >= (unsigned)o) ? sh[threadIdx.x - o] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < ntiles) tile_sum[i] = carry + sh[threadIdx.x] - v;
-    const int32_t tot = sh[1023];
-    __syncthreads();
-    carry += tot;
+  if (base + 8 <= n && ((uintptr_t)out & 15u) == 0) {
+    *reinterpret_cast<int4 *>(out + base) = make_int4(o8[0], o8[1], o8[2], o8[3]);
+    *reinterpret_cast<int4 *>(out + base + 4) = make_int4(o8[4], o8[5], o8[6], o8[7]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      if (base + m < n) out[base + m] = o8[m];
   }
 }
-__global__ __launch_bounds__(256) void k_scan_add(int32_t *out, const int32_t *tile_sum, int n) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] += tile_sum[i >> 10];
+
+static void scan_table(const lbdem_dem_work *w, int T, hipStream_t s) {
+  const unsigned nt = (unsigned)((T + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(256), 0, s, w->count, w->tile_sum, T);
+  hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(256), 0, s, w->count, w->offset, w->tile_sum, T);
 }
 
 __global__ __launch_bounds__(256) void k_bp_scatter(const lbdem_dem_params P, const int32_t *hash,
@@ -1551,7 +1580,6 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
                           double *moment, hipStream_t s, const char *who, const double *rad_touch = nullptr) {
   const int n = P->n;
   const unsigned nb = (unsigned)((n + 255) / 256);
-  const int ntiles = (P->T + 1023) / 1024;
   const bool fine = rad_touch && P->fine_cell > 0.0;
   // with the second grid the main grid needs its hashes only (the release kernel and the
   // candidate filter read them); the second grid is built in the same table
@@ -1561,9 +1589,7 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
     lbdem_dem_params Pf = *P;
     Pf.cell = P->fine_cell;
     hipLaunchKernelGGL(k_bp_count, dim3(nb), dim3(256), 0, s, Pf, pos, w->fhash, w->count, nullptr, nullptr);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, w->count, w->offset, w->tile_sum, P->T);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, ntiles);
-    hipLaunchKernelGGL(k_scan_add, dim3((P->T + 255) / 256), dim3(256), 0, s, w->offset, w->tile_sum, P->T);
+    scan_table(w, P->T, s);
     hipLaunchKernelGGL(k_bp_scatter, dim3(nb), dim3(256), 0, s, Pf, w->fhash, w->offset, w->current, w->pid);
     hipLaunchKernelGGL(k_bp_sort_state, dim3(nb), dim3(256), 0, s, Pf, pos, rad_touch, w->pid, w->spos);
     const unsigned split = (unsigned)std::min(16, std::max(1, (int)((2048 + nb - 1) / nb)));
@@ -1577,9 +1603,7 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
     LBDEM_CHECK_LAUNCH(who);
     return LBDEM_OK;
   }
-  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, s, w->count, w->offset, w->tile_sum, P->T);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, ntiles);
-  hipLaunchKernelGGL(k_scan_add, dim3((P->T + 255) / 256), dim3(256), 0, s, w->offset, w->tile_sum, P->T);
+  scan_table(w, P->T, s);
   hipLaunchKernelGGL(k_bp_scatter, dim3(nb), dim3(256), 0, s, *P, w->hash, w->offset, w->current, w->pid);
   if (rad_touch) {
     // touching candidates only; each particle's candidates are split over more lanes when
