@@ -736,14 +736,26 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
 // The touching probe of the 27-cell rule over particles in bucket order (spos): lane t takes
 // the t-th particle of the bucket order, so a wavefront's particles share most of their buckets
 // and read their candidates from neighbouring 32-byte records.  The walk is latency-bound, so
-// blockIdx.y (three workgroups) splits each particle's 27 buckets into the three z slabs of 9,
-// not their entries; a lane loads its slab's 9 bucket headers together, then walks the 9
-// buckets' entries as one range, four loads at a time (eight measured no faster).
+// blockIdx.y (three workgroups) splits each particle's buckets, not their entries; a lane loads
+// its share's bucket headers together, then walks those buckets' entries as one range, four
+// loads at a time (eight measured no faster).  Each touching pair is listed once, tested with the
+// lower index first (as the resolve kernels and the oracle evaluate it):
+//  - HALF (T >= 256): a half shell.  hash3 keeps each cell coordinate modulo a power of two of at
+//    least 4 then, so the 27 probed cells have distinct hashes and j's bucket is i's cell + d for
+//    exactly one d, and i's is j's cell - d.  A particle walks the 13 cells with d > 0 (z first,
+//    then y, then x) and its own bucket's entries after its own position t0: every pair is found
+//    from exactly one side.  The three workgroups take 5, 5 and 4 of these 14 buckets.
+//  - otherwise (hashes may alias within the 27): each workgroup takes a z slab of 9 deduplicated
+//    buckets and walks only their entries after t0, so a pair is found by whichever of its two
+//    particles comes first in bucket order (the aliasing is symmetric: j's cell aliases i's
+//    cell + d exactly when i's aliases j's - d).
+template <bool HALF>
 __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_params P, const int32_t *count,
                                                                const int32_t *offset, const int32_t *pid,
                                                                const double4 *spos, int32_t *pairs, int64_t cap,
                                                                unsigned long long *npairs) {
   constexpr int kStage = pair_stage_size<27>();
+  constexpr int NB = HALF ? 5 : 9;  // buckets per workgroup
   __shared__ int2 s_pairs[kStage];
   __shared__ int s_np;
   __shared__ unsigned long long s_base;
@@ -756,20 +768,51 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_p
     const double4 q = spos[t0];
     const double xi[3] = {q.x, q.y, q.z};
     const double ri = q.w;
-    int32_t c[3], h[27];
+    int32_t c[3], hs[NB];
+    int self = -1;  // the bucket among hs holding this particle (walked after t0 only)
     cell_of(P, xi, c);
-    probe_cells<27>(P, xi, c, h);
-    // this workgroup's 9 buckets, k = 3 m + part (the cells at z offset part - 1): all headers
-    // first, then the entries of the 9 buckets as one range, four loads at a time
-    int32_t beg[9], pre[9];
+    if constexpr (HALF) {
+      const int32_t x = c[0], y = c[1], z = c[2];
+      if (part == 0) {
+        hs[0] = hash3(x - 1, y - 1, z + 1, P.T);
+        hs[1] = hash3(x, y - 1, z + 1, P.T);
+        hs[2] = hash3(x + 1, y - 1, z + 1, P.T);
+        hs[3] = hash3(x - 1, y, z + 1, P.T);
+        hs[4] = hash3(x, y, z, P.T);
+        self = 4;
+      } else if (part == 1) {
+        hs[0] = hash3(x, y, z + 1, P.T);
+        hs[1] = hash3(x + 1, y, z + 1, P.T);
+        hs[2] = hash3(x - 1, y + 1, z + 1, P.T);
+        hs[3] = hash3(x, y + 1, z + 1, P.T);
+        hs[4] = hash3(x + 1, y, z, P.T);
+      } else {
+        hs[0] = hash3(x + 1, y + 1, z + 1, P.T);
+        hs[1] = hash3(x - 1, y + 1, z, P.T);
+        hs[2] = hash3(x, y + 1, z, P.T);
+        hs[3] = hash3(x + 1, y + 1, z, P.T);
+        hs[4] = -1;
+      }
+    } else {
+      int32_t h[27];
+      probe_cells<27>(P, xi, c, h);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) hs[m] = part == 0 ? h[3 * m] : (part == 1 ? h[3 * m + 1] : h[3 * m + 2]);
+    }
+    // all headers first, then the entries of the buckets as one range
+    int32_t beg[NB], pre[NB];
     int total = 0;
 #pragma unroll
-    for (int m = 0; m < 9; ++m) {
-      const int32_t hk = part == 0 ? h[3 * m] : (part == 1 ? h[3 * m + 1] : h[3 * m + 2]);
+    for (int m = 0; m < NB; ++m) {
       int32_t b = 0, cn = 0;
-      if (hk >= 0) {
-        b = offset[hk];
-        cn = count[hk];
+      if (hs[m] >= 0) {
+        b = offset[hs[m]];
+        cn = count[hs[m]];
+        if (!HALF || m == self) {  // entries after t0 only
+          const int32_t e1 = b + cn;
+          b = max(b, t0 + 1);
+          cn = max(e1 - b, 0);
+        }
       }
       pre[m] = total;
       beg[m] = b - total;  // entry e of the range, pre[m] <= e < pre[m] + cn, is at beg[m] + e
@@ -785,7 +828,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_p
         if (e < total) {
           int32_t b = beg[0];  // the last bucket starting at or before e holds it
 #pragma unroll
-          for (int m = 1; m < 9; ++m)
+          for (int m = 1; m < NB; ++m)
             if (e >= pre[m]) b = beg[m];
           jj[u] = pid[b + e];
           qq[u] = spos[b + e];
@@ -793,9 +836,11 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_p
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (jj[u] <= i) continue;
+        const int j = jj[u];
+        if (j < 0) continue;
         const double xj[3] = {qq[u].x, qq[u].y, qq[u].z};
-        if (touching(P, xi, ri, xj, qq[u].w)) stage_pair<kStage>(i, jj[u], s_pairs, &s_np, pairs, cap, npairs);
+        const bool hit = i < j ? touching(P, xi, ri, xj, qq[u].w) : touching(P, xj, qq[u].w, xi, ri);
+        if (hit) stage_pair<kStage>(min(i, j), max(i, j), s_pairs, &s_np, pairs, cap, npairs);
       }
     }
   }
@@ -1621,8 +1666,12 @@ static int run_broadphase(const lbdem_dem_params *P, const double *pos, const lb
     const double4 *sp = P->probe27 ? reinterpret_cast<const double4 *>(w->spos) : nullptr;
     if (sp) {
       hipLaunchKernelGGL(k_bp_sort_state, dim3(nb), dim3(256), 0, s, *P, pos, rad_touch, w->pid, w->spos);
-      hipLaunchKernelGGL(k_bp_probe_touch_sorted, dim3(nb, 3u), dim3(256), 0, s, *P, w->count,
-                         w->offset, w->pid, sp, w->pairs, w->pair_cap, w->npairs);
+      if (P->T >= 256)
+        hipLaunchKernelGGL(k_bp_probe_touch_sorted<true>, dim3(nb, 3u), dim3(256), 0, s, *P, w->count,
+                           w->offset, w->pid, sp, w->pairs, w->pair_cap, w->npairs);
+      else
+        hipLaunchKernelGGL(k_bp_probe_touch_sorted<false>, dim3(nb, 3u), dim3(256), 0, s, *P, w->count,
+                           w->offset, w->pid, sp, w->pairs, w->pair_cap, w->npairs);
     } else if (P->probe27)
       hipLaunchKernelGGL(k_bp_probe_touch<27>, dim3(nb, split), dim3(256), 0, s, *P, pos, rad_touch, w->count,
                          w->offset, w->pid, w->pairs, w->pair_cap, w->npairs);

@@ -170,22 +170,18 @@ def _dense_bed(n, ext, seed):
     return rad, 2.5 * 4.0 / 3.0 * np.pi * rad ** 3, pos
 
 
-def _boxed_wave_fraction(pos, cell, T):
-    """Share of 64-particle wavefronts (particles in bucket order) whose cells lie within a box of
-    at most 343 cells, i.e. that the touching probe walks together."""
-    c = np.floor(pos / cell).astype(np.int64)
-    h = np.array([O.morton3d32(*v) for v in c]) & (T - 1)
-    c = c[np.argsort(h, kind="stable")]
-    boxed = [np.prod(c[w:w + 64].max(0) - c[w:w + 64].min(0) + 3) <= 343 for w in range(0, len(c), 64)]
-    return float(np.mean(boxed))
-
-
+@pytest.mark.parametrize("table", [None, 256, 32])
 @pytest.mark.parametrize("periodic", [False, True])
-def test_dem_fine_grid_probe27_matches_oracle(periodic):
+def test_dem_fine_grid_probe27_matches_oracle(periodic, table):
     """6,000 grains seeded overlapping in a 12 x 12 x 6 box (about 7 per cell), broad-phase cells
-    one maximum diameter wide with the 27-cell probe, as in the unresolved bed: most wavefronts
-    of the general path's touching probe walk their box of cells together, the rest per lane.
-    100 substeps of soft linear contacts (and a periodic x axis) against the oracle: positions,
+    one maximum diameter wide with the 27-cell probe, as in the unresolved bed.  The touching
+    probe lists each pair from one side only: with the table of the reference's size rule (2^20
+    entries) and with 256 entries (cell coordinates kept modulo 8, 8 and 4, the least for which
+    the 27 probed hashes stay distinct) it walks the half shell of 13 cells and its own bucket;
+    with 32 entries (coordinates modulo 4, 4 and 2, so the 27 cells share 18 buckets) it walks
+    the deduplicated buckets' entries after its own position.  The oracle runs with the same
+    table (which cells alias decides which history slots the release keeps).  100, 40 and 10
+    substeps of soft linear contacts (and a periodic x axis) against the oracle: positions,
     velocities and the contact sets."""
     rad, mass, pos = _dense_bed(6000, (12.0, 12.0, 6.0), seed=21)
     vel = np.zeros_like(pos)
@@ -193,11 +189,17 @@ def test_dem_fine_grid_probe27_matches_oracle(periodic):
     cfg = make_config(model="linear", box=(0.0, 12.0, 0.0, 12.0, 0.0, 6.0), dt=0.1, g=(0.0, 0.0, -1e-5), K=64,
                       periodic=(periodic, False, False), kn=1e-2, kt=1e-2, zeta=0.3, mu=0.5)
     gpu = gpu_solver(cfg, rad, mass, pos, vel, path=2, cell_size=cell, probe27=True)
+    if table is not None:
+        gpu.bpcd.table_size = table
+        gpu.bpcd._ws = None
+        gpu._cached = None
     orc = oracle_solver(cfg, rad, mass, pos, vel, cell=cell, probe27=True)
-    assert _boxed_wave_fraction(pos, cell, orc.params.T) > 0.5
-    for _ in range(2):
-        gpu.run_substeps(50)
-        orc.step(50)
+    if table is not None:
+        orc.params.T = table
+    chunks = {None: (50, 50), 256: (20, 20), 32: (5, 5)}[table]
+    for m in chunks:
+        gpu.run_substeps(m)
+        orc.step(m)
     torch.cuda.synchronize()
     assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
     assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
@@ -207,6 +209,7 @@ def test_dem_fine_grid_probe27_matches_oracle(periodic):
     oact = orc["cf_active"].astype(bool)
     ref = sorted(zip(np.repeat(np.arange(orc.n), K)[oact].tolist(), orc["cf_j"][oact].tolist()))
     assert got == ref and len(ref) > 10000
+    assert gpu._cached[0].T == (table or 1 << 20)
     gpu.check()
 
 
