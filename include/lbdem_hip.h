@@ -430,7 +430,10 @@ typedef struct lbdem_dem_work {
   double *spos;                      /* 4n, may be NULL: x, y, z and radius of each particle in
                                         bucket order, written by the general path of lbdem_dem_run
                                         so that its probe walks particles and candidates in space
-                                        order (NULL: particle order) */
+                                        order (NULL: particle order).  Both broad-phase entry
+                                        points also keep each particle's bucket slot in its first
+                                        4n bytes between the count and the scatter (NULL: the
+                                        scatter counts again, in current) */
   int32_t *fhash;                    /* n, with fine_cell > 0: cell hashes on the second grid */
   const int32_t *large;              /* nlarge, with fine_cell > 0: indices of the particles of
                                         radius > large_r */

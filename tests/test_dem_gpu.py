@@ -45,6 +45,27 @@ def test_broadphase_bit_exact(fixture, probe27):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("probe27", [False, True])
+def test_broadphase_without_spos_bit_exact(probe27):
+    """With no spos buffer the scatter counts the bucket slots again (in `current`) instead of
+    keeping those of the count kernel; same candidate set."""
+    from src.dem3d.bpcd import BPCD
+    a = read_p4p("sill_5.p4p")
+    rad, pos = a[:, 2], a[:, 4:7]
+    dmin = pos.min(0) - rad.max()
+    dmax = pos.max(0) + rad.max()
+    cell = 2.0 * rad.max() if probe27 else 4.0 * rad.max()
+    bp = BPCD.create(len(pos), rad.max(), dmin, dmax, cell_size=cell, probe27=probe27)
+    n = len(pos)
+    _, w = bp.workspace(n, max(64 * n, 1024), torch.device("cuda"))
+    w.spos = None
+    ref = O.broadphase(pos, dmin, cell, bp.table_size, probe27=probe27)
+    for _ in range(2):  # the second call finds the counters cleared
+        got = bp.detect_collision(torch.tensor(pos, dtype=torch.float64, device="cuda")).cpu().numpy()
+        got = got[np.lexsort((got[:, 1], got[:, 0]))]
+        assert np.array_equal(got, ref)
+
+
 def test_broadphase_empty_and_single():
     from src.dem3d.bpcd import BPCD
     bp = BPCD.create(1, 0.1, [0, 0, 0], [1, 1, 1])
@@ -210,6 +231,38 @@ def test_dem_fine_grid_probe27_matches_oracle(periodic, table):
     ref = sorted(zip(np.repeat(np.arange(orc.n), K)[oact].tolist(), orc["cf_j"][oact].tolist()))
     assert got == ref and len(ref) > 10000
     assert gpu._cached[0].T == (table or 1 << 20)
+    gpu.check()
+
+
+@pytest.mark.parametrize("probe27", [False, True])
+def test_dem_without_spos_matches_oracle(probe27):
+    """The grid-wide path with no spos buffer: the scatter counts in `current`, the release
+    kernel clears it with the counts, and the touching probe walks particles in index order
+    (the 8-cell per-lane or staged walk, or the 27-cell per-lane walk).  The first substep runs
+    with spos, the other 59 without; trajectories and contact sets against the oracle."""
+    rad, mass, pos = _dense_bed(4000, (10.0, 10.0, 5.0), seed=23)
+    vel = np.zeros_like(pos)
+    cell = float((2.0 if probe27 else 4.0) * rad.max())
+    cfg = make_config(model="linear", box=(0.0, 10.0, 0.0, 10.0, 0.0, 5.0), dt=0.1, g=(0.0, 0.0, -1e-5), K=64,
+                      kn=1e-2, kt=1e-2, zeta=0.3, mu=0.5)
+    gpu = gpu_solver(cfg, rad, mass, pos, vel, path=2, cell_size=cell, probe27=probe27)
+    orc = oracle_solver(cfg, rad, mass, pos, vel, cell=cell, probe27=probe27)
+    gpu.run_substeps(1)
+    orc.step(1)
+    gpu._cached[2].spos = None
+    for m in (29, 30):
+        gpu.run_substeps(m)
+        orc.step(m)
+    torch.cuda.synchronize()
+    assert gpu._cached[2].spos is None and gpu._cached[0].fine_cell == 0.0
+    assert np.abs(gpu.gf.position.to_numpy() - orc["pos"]).max() < 1e-9
+    assert np.abs(gpu.gf.velocity.to_numpy() - orc["vel"]).max() < 1e-7 * max(1.0, np.abs(orc["vel"]).max())
+    K = gpu.K
+    act = gpu._cf_active.cpu().numpy().astype(bool)
+    got = sorted(zip(np.repeat(np.arange(gpu.n), K)[act].tolist(), gpu._cf_j.cpu().numpy()[act].tolist()))
+    oact = orc["cf_active"].astype(bool)
+    ref = sorted(zip(np.repeat(np.arange(orc.n), K)[oact].tolist(), orc["cf_j"][oact].tolist()))
+    assert got == ref and len(ref) > 5000
     gpu.check()
 
 
