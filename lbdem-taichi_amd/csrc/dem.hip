@@ -43,9 +43,10 @@ __device__ __forceinline__ void cell_of(const lbdem_dem_params &P, const double 
 }
 
 // hashes of the probed cells of a particle in cell c at x: the octant's 8 cells
-// (bpcd.py:114-152) or all 27; duplicates (hash aliasing) become -1 so every bucket is visited
-// once.  NP is a template argument so every index into h is a compile-time constant.
-template <int NP>
+// (bpcd.py:114-152) or all 27; with DEDUP, duplicates (hash aliasing) become -1 so every bucket
+// is visited once (callers that only ask whether a hash is among them skip that step).  NP is a
+// template argument so every index into h is a compile-time constant.
+template <int NP, bool DEDUP = true>
 __device__ __forceinline__ void probe_cells(const lbdem_dem_params &P, const double *x, const int32_t (&c)[3],
                                             int32_t (&h)[NP]) {
   if constexpr (NP == 8) {
@@ -67,11 +68,13 @@ __device__ __forceinline__ void probe_cells(const lbdem_dem_params &P, const dou
 #pragma unroll
     for (int t = 0; t < 27; ++t) h[t] = hash3(c[0] + t / 9 - 1, c[1] + (t / 3) % 3 - 1, c[2] + t % 3 - 1, P.T);
   }
+  if constexpr (DEDUP) {
 #pragma unroll
-  for (int k = 1; k < NP; ++k)
+    for (int k = 1; k < NP; ++k)
 #pragma unroll
-    for (int e = 0; e < k; ++e)
-      if (h[e] == h[k]) h[k] = -1;  // a later match compares against -1 only, which changes nothing
+      for (int e = 0; e < k; ++e)
+        if (h[e] == h[k]) h[k] = -1;  // a later match compares against -1 only, which changes nothing
+  }
 }
 
 // atomicAdd(ptr + h, 1) for each active lane; lanes of the wave sharing h are combined into one
@@ -878,7 +881,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_sorted(const lbdem_dem_p
 __device__ __forceinline__ bool probed_by(const lbdem_dem_params &P, const double *xi, int32_t hj) {
   int32_t c[3], h[8];
   cell_of(P, xi, c);
-  probe_cells<8>(P, xi, c, h);
+  probe_cells<8, false>(P, xi, c, h);
   bool in = false;
 #pragma unroll
   for (int k = 0; k < 8; ++k) in |= (h[k] == hj);
@@ -910,7 +913,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch_fine(const lbdem_dem_par
       {
         int32_t cm[3];
         cell_of(P, xi, cm);
-        probe_cells<8>(P, xi, cm, hm);
+        probe_cells<8, false>(P, xi, cm, hm);
       }
 #pragma unroll
       for (int k = 0; k < 27; ++k) {
@@ -1005,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_dem_release(const lbdem_dem_params P, c
   int32_t c[3];
   cell_of(P, xi, c);
   int32_t h[NP];
-  probe_cells<NP>(P, xi, c, h);
+  probe_cells<NP, false>(P, xi, c, h);
   const double ri = S.radius[i];
   for (int s = 0; s < m; ++s) {
     const int off = i * K + s;
