@@ -124,12 +124,16 @@ __global__ __launch_bounds__(256) void k_eq_spread(const IbmArgs a) {
 // cells x 5 fields with global atomics (14 million per step for 200,000 grains, 0.46 ms); in a
 // spatial order without the LDS stage, concurrent wavefronts hit the same cells and it is
 // slower still (0.63 ms).  A workgroup whose union box exceeds kSpreadCells adds directly.
+// Each wavefront takes the workgroup's grains w, w + 4, ... one after another; their lattice
+// positions, radii and velocities are staged in LDS by the pass that finds the union box, and
+// the next grain's box cells' flags are loaded while the current grain is evaluated.
 constexpr int kSpreadGrains = 128;  // 256 measured slower (0.31 -> 0.41 ms per update_coupling)
 constexpr int kSpreadCells = 1536;
 static_assert(kSpreadGrains <= 256, "the union box is built with one thread per grain of a 256-thread workgroup");
 
 __global__ __launch_bounds__(256) void k_eq_spread_ordered(const IbmArgs a) {
   __shared__ float s_acc[5][kSpreadCells];
+  __shared__ double s_g[7][kSpreadGrains];  // per grain: lattice position, radius, velocity
   __shared__ int s_lo[3], s_hi[3];
   const int g0 = blockIdx.x * kSpreadGrains;
   const int ng = min(kSpreadGrains, a.np - g0);
@@ -139,8 +143,16 @@ __global__ __launch_bounds__(256) void k_eq_spread_ordered(const IbmArgs a) {
   }
   __syncthreads();
   if ((int)threadIdx.x < ng) {
+    const int p = a.order[g0 + threadIdx.x];
     double c[3];
-    lattice_pos(a, a.order[g0 + threadIdx.x], c[0], c[1], c[2]);
+    lattice_pos(a, p, c[0], c[1], c[2]);
+    s_g[0][threadIdx.x] = c[0];
+    s_g[1][threadIdx.x] = c[1];
+    s_g[2][threadIdx.x] = c[2];
+    s_g[3][threadIdx.x] = a.rad[p];
+    s_g[4][threadIdx.x] = a.vel[3 * p];
+    s_g[5][threadIdx.x] = a.vel[3 * p + 1];
+    s_g[6][threadIdx.x] = a.vel[3 * p + 2];
     const int N[3] = {a.nx, a.ny, a.nz};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -165,21 +177,31 @@ __global__ __launch_bounds__(256) void k_eq_spread_ordered(const IbmArgs a) {
   const double pi = 3.14159265358979323846;
   const double Vl = a.dx * a.dx * a.dx, s = a.dt / a.dx;
   if (any) {
-    for (int q = (int)(threadIdx.x >> 6); q < ng; q += 4) {  // uniform over the wavefront
-      const int p = a.order[g0 + q];
-      double xc, yc, zc;
-      lattice_pos(a, p, xc, yc, zc);
-      const BoxLane b = box_lane(a, xc, yc, zc);
+    const int w = (int)(threadIdx.x >> 6);
+    BoxLane bn{};
+    uint8_t fn = 0;
+    if (w < ng) {
+      bn = box_lane(a, s_g[0][w], s_g[1][w], s_g[2][w]);
+      fn = bn.in ? a.flags[bn.c] : 0;
+    }
+    for (int q = w; q < ng; q += 4) {  // uniform over the wavefront
+      const BoxLane b = bn;
+      const uint8_t fl = fn;
+      if (q + 4 < ng) {
+        bn = box_lane(a, s_g[0][q + 4], s_g[1][q + 4], s_g[2][q + 4]);
+        fn = bn.in ? a.flags[bn.c] : 0;
+      }
+      const double xc = s_g[0][q], yc = s_g[1][q], zc = s_g[2][q];
       double wb = -1.0;
-      if (b.in && (a.flags[b.c] & LBDEM_FLAG_ACTIVE)) wb = kernel_mirror(a, xc, yc, zc, b.i, b.j, b.k);
+      if (b.in && (fl & LBDEM_FLAG_ACTIVE)) wb = kernel_mirror(a, xc, yc, zc, b.i, b.j, b.k);
       const double denom = wave_sum(wb >= 0.0 ? wb : 0.0);
       if (denom < 1e-30 || wb < 0.0) continue;
-      const double R = a.rad[p];
+      const double R = s_g[3][q];
       const double Vp = 4.0 / 3.0 * pi * R * R * R;
       const double wn = wb / denom, e = wn * Vp / Vl;
       if (e == 0.0) continue;
-      const float v[5] = {(float)e, (float)(a.vel[3 * p] * s * e), (float)(a.vel[3 * p + 1] * s * e),
-                          (float)(a.vel[3 * p + 2] * s * e), (float)wn};
+      const float v[5] = {(float)e, (float)(s_g[4][q] * s * e), (float)(s_g[5][q] * s * e),
+                          (float)(s_g[6][q] * s * e), (float)wn};
       if (lds) {
         const int l = ((b.i - lo0) * ex1 + (b.j - lo1)) * ex2 + (b.k - lo2);
 #pragma unroll
