@@ -1251,21 +1251,22 @@ __global__ __launch_bounds__(256) void k_dem_resolve_new(const lbdem_dem_params 
   for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < np; p += (int64_t)gridDim.x * 256)
     if (need_new[p]) resolve_new(P, S, pairs[2 * p], pairs[2 * p + 1]);
 }
+// walls + integration and the compaction of particle i's history slots (independent: the
+// slots are not read by the integration), one launch; the pair list is emptied for the next
+// substep after its length is checked against the capacity
 __global__ __launch_bounds__(256) void k_dem_integrate(const lbdem_dem_params P, const lbdem_dem_state S,
-                                                       const unsigned long long *npairs, int64_t cap) {
+                                                       unsigned long long *npairs, int64_t cap) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i == 0 && *npairs > (unsigned long long)cap) atomicMax(S.status + 4, (int)min(*npairs, 0x7fffffffull));
+  if (i == 0) {
+    const unsigned long long m = *npairs;
+    if (m > (unsigned long long)cap) atomicMax(S.status + 4, (int)min(m, 0x7fffffffull));
+    *npairs = 0ull;
+  }
   if (i >= P.n) return;
   const unsigned w0 = load_wall_mask(S, i);
   unsigned w = w0;
   walls_and_integrate(P, S, i, w);
   store_wall_mask(S, i, w0, w);
-}
-__global__ __launch_bounds__(256) void k_dem_compact(const lbdem_dem_params P, const lbdem_dem_state S,
-                                                     unsigned long long *npairs) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i == 0 && npairs) *npairs = 0ull;
-  if (i >= P.n) return;
   compact_slots(P, S, i);
 }
 
@@ -1791,7 +1792,6 @@ int lbdem_dem_run(const lbdem_dem_params *P, const lbdem_dem_state *S, const lbd
     hipLaunchKernelGGL(k_dem_resolve_new, dim3(pb), dim3(256), 0, s, *P, *S, w->pairs, w->npairs, w->pair_cap,
                        w->need_new);
     hipLaunchKernelGGL(k_dem_integrate, dim3(nb), dim3(256), 0, s, *P, *S, w->npairs, w->pair_cap);
-    hipLaunchKernelGGL(k_dem_compact, dim3(nb), dim3(256), 0, s, *P, *S, w->npairs);
     LBDEM_CHECK_LAUNCH("lbdem_dem_run");
   }
   return LBDEM_OK;
