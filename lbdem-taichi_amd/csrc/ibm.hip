@@ -34,19 +34,22 @@ __device__ __forceinline__ double threedelta(double r) {
   return a;
 }
 
+// threedelta(sqrt(r2)) with the support tested on r2 first: for r2 > 2.25 the rounded square
+// root is at least 1.5, where threedelta is 0 (at exactly 1.5 its second branch gives
+// (0.5 - sqrt(0.25)) / 6 = 0), so the value is the same and the square roots are skipped
+__device__ __forceinline__ double threedelta_sq(double r2) { return r2 > 2.25 ? 0.0 : threedelta(sqrt(r2)); }
+
 // _kernel_with_mirror (src/unlbdem/eqlattice.py:468-528)
 __device__ __forceinline__ double kernel_mirror(const IbmArgs &a, double xc, double yc, double zc, int i, int j, int k) {
   const double s = 1.5;
-  auto d3 = [&](double x, double y, double z) {
-    return sqrt((x - i) * (x - i) + (y - j) * (y - j) + (z - k) * (z - k));
-  };
-  double w = threedelta(d3(xc, yc, zc));
-  if (xc < s) w += threedelta(d3(-xc, yc, zc));
-  if (xc > (double)(a.nx - 1) - s) w += threedelta(d3(2.0 * (a.nx - 1) - xc, yc, zc));
-  if (yc < s) w += threedelta(d3(xc, -yc, zc));
-  if (yc > (double)(a.ny - 1) - s) w += threedelta(d3(xc, 2.0 * (a.ny - 1) - yc, zc));
-  if (zc < s) w += threedelta(d3(xc, yc, -zc));
-  if (zc > (double)(a.nz - 1) - s) w += threedelta(d3(xc, yc, 2.0 * (a.nz - 1) - zc));
+  auto d2 = [&](double x, double y, double z) { return (x - i) * (x - i) + (y - j) * (y - j) + (z - k) * (z - k); };
+  double w = threedelta_sq(d2(xc, yc, zc));
+  if (xc < s) w += threedelta_sq(d2(-xc, yc, zc));
+  if (xc > (double)(a.nx - 1) - s) w += threedelta_sq(d2(2.0 * (a.nx - 1) - xc, yc, zc));
+  if (yc < s) w += threedelta_sq(d2(xc, -yc, zc));
+  if (yc > (double)(a.ny - 1) - s) w += threedelta_sq(d2(xc, 2.0 * (a.ny - 1) - yc, zc));
+  if (zc < s) w += threedelta_sq(d2(xc, yc, -zc));
+  if (zc > (double)(a.nz - 1) - s) w += threedelta_sq(d2(xc, yc, 2.0 * (a.nz - 1) - zc));
   return w;
 }
 
