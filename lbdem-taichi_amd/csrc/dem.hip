@@ -130,7 +130,8 @@ __global__ __launch_bounds__(256) void k_bp_count(const lbdem_dem_params P, cons
 
 // exclusive scan of count[T] -> offset[T] in two passes over tiles of kScanTile entries: the
 // tiles' sums, then each tile's scan plus the sums of the tiles before it (read back from L2 by
-// every workgroup; at most 2048 of them for T = 2^22).  Eight entries per lane, as two 16-byte
+// every workgroup; at most kScanInlineTiles of them, 2^22 entries; a larger table first scans
+// the tile sums in one workgroup, k_scan_sums, so the work stays linear in T).  Eight entries per lane, as two 16-byte
 // accesses when the arrays allow it (the branch is uniform: a lane's first entry is a multiple of
 // 8 entries into the array).  The three-pass form (tile scans, a one-workgroup scan of the sums,
 // an add pass) took 17 us per substep on the 200,000-grain bed's 2^21-entry table.
@@ -164,12 +165,45 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const int32_t *in, int32_t 
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+constexpr int kScanInlineTiles = 2048;
+
+// exclusive scan of tile_sum[0, nt) in place, one workgroup
+__global__ __launch_bounds__(1024) void k_scan_sums(int32_t *tile_sum, int nt) {
+  __shared__ int32_t sh[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int32_t carry = 0;
+  for (int base = 0; base < nt; base += 1024) {
+    const int k = base + (int)threadIdx.x;
+    const int32_t v = k < nt ? tile_sum[k] : 0;
+    int32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wave] = x;
+    __syncthreads();
+    int32_t pre = carry;
+    for (int w = 0; w < wave; ++w) pre += sh[w];
+    int32_t tot = 0;
+    for (int w = 0; w < 16; ++w) tot += sh[w];
+    if (k < nt) tile_sum[k] = pre + x - v;
+    carry += tot;
+    __syncthreads();  // sh is rewritten by the next chunk
+  }
+}
+
+// prescanned: tile_sum holds the exclusive prefix of the tile sums (k_scan_sums)
 __global__ __launch_bounds__(256) void k_scan_down(const int32_t *in, int32_t *out, const int32_t *tile_sum,
-                                                   int n) {
+                                                   int n, int prescanned) {
   __shared__ int32_t sh[8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int32_t c = 0;
-  for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) c += tile_sum[k];
+  if (prescanned) {
+    if (threadIdx.x == 0) c = tile_sum[blockIdx.x];
+  } else {
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) c += tile_sum[k];
+  }
   c = wave_sum(c);
   const int base = blockIdx.x * kScanTile + threadIdx.x * 8;
   int32_t v[8];
@@ -204,8 +238,10 @@ __global__ __launch_bounds__(256) void k_scan_down(const int32_t *in, int32_t *o
 
 static void scan_table(const lbdem_dem_work *w, int T, hipStream_t s) {
   const unsigned nt = (unsigned)((T + kScanTile - 1) / kScanTile);
+  const int prescan = nt > (unsigned)kScanInlineTiles ? 1 : 0;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(256), 0, s, w->count, w->tile_sum, T);
-  hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(256), 0, s, w->count, w->offset, w->tile_sum, T);
+  if (prescan) hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, w->tile_sum, (int)nt);
+  hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(256), 0, s, w->count, w->offset, w->tile_sum, T, prescan);
 }
 
 __global__ __launch_bounds__(256) void k_bp_scatter(const lbdem_dem_params P, const int32_t *hash,

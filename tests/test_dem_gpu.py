@@ -66,6 +66,25 @@ def test_broadphase_without_spos_bit_exact(probe27):
         assert np.array_equal(got, ref)
 
 
+def test_broadphase_large_table_bit_exact():
+    """A table of 2^23 entries (4,096 scan tiles, more than the 2,048 whose sums every tile adds
+    up itself): the tile sums are scanned by one workgroup first.  Same candidate set as the
+    oracle on the same table, on two calls in a row."""
+    from src.dem3d.bpcd import BPCD
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(0.0, 1.0, (20000, 3))
+    rad = rng.uniform(0.002, 0.006, 20000)
+    dmin = pos.min(0) - rad.max()
+    cell = 4.0 * rad.max()
+    T = 1 << 23
+    bp = BPCD(len(pos), T, rad.max(), dmin, cell_size=cell)
+    ref = O.broadphase(pos, dmin, cell, T)
+    for _ in range(2):
+        got = bp.detect_collision(torch.tensor(pos, dtype=torch.float64, device="cuda")).cpu().numpy()
+        got = got[np.lexsort((got[:, 1], got[:, 0]))]
+        assert np.array_equal(got, ref) and len(ref) > 1000
+
+
 def test_broadphase_empty_and_single():
     from src.dem3d.bpcd import BPCD
     bp = BPCD.create(1, 0.1, [0, 0, 0], [1, 1, 1])
