@@ -131,10 +131,11 @@ __global__ __launch_bounds__(256) void k_bp_count(const lbdem_dem_params P, cons
 // exclusive scan of count[T] -> offset[T] in two passes over tiles of kScanTile entries: the
 // tiles' sums, then each tile's scan plus the sums of the tiles before it (read back from L2 by
 // every workgroup; at most kScanInlineTiles of them, 2^22 entries; a larger table first scans
-// the tile sums in one workgroup, k_scan_sums, so the work stays linear in T).  Eight entries per lane, as two 16-byte
-// accesses when the arrays allow it (the branch is uniform: a lane's first entry is a multiple of
-// 8 entries into the array).  The three-pass form (tile scans, a one-workgroup scan of the sums,
-// an add pass) took 17 us per substep on the 200,000-grain bed's 2^21-entry table.
+// the tile sums in one workgroup, k_scan_sums, so the work stays linear in T).  Eight entries
+// per lane, as two 16-byte accesses when the arrays allow it (the branch is uniform: a lane's
+// first entry is a multiple of 8 entries into the array).  The three-pass form (tile scans, a
+// one-workgroup scan of the sums, an add pass) took 17 us per substep on the 200,000-grain
+// bed's 2^21-entry table.
 constexpr int kScanTile = 2048;
 
 __device__ __forceinline__ void scan_load8(const int32_t *in, int base, int n, int32_t v[8]) {
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(256) void k_bp_scatter(const lbdem_dem_params P, co
   const int32_t slot = agg_add_one(current, h, active);
   if (active) pid[offset[h] + slot] = i;
 }
-// the scatter with the slots k_bp_count kept (12.8 -> 6 us per substep on the 200,000-grain bed)
+// the scatter with the slots k_bp_count kept (12.8 -> 5.2 us per substep on the 200,000-grain bed)
 __global__ __launch_bounds__(256) void k_bp_scatter_slots(const lbdem_dem_params P, const int32_t *hash,
                                                           const int32_t *offset, const int32_t *slot, int32_t *pid) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -791,7 +792,7 @@ __global__ __launch_bounds__(256) void k_bp_probe_touch(const lbdem_dem_params P
 // loads at a time (eight measured no faster).  Each touching pair is listed once, tested with the
 // lower index first (as the resolve kernels and the oracle evaluate it):
 //  - HALF (T >= 256): a half shell.  hash3 keeps each cell coordinate modulo a power of two of at
-//    least 4 then, so the 27 probed cells have distinct hashes and j's bucket is i's cell + d for
+//    least 4 then (from T = 64 on, in fact), so the 27 probed cells have distinct hashes and j's bucket is i's cell + d for
 //    exactly one d, and i's is j's cell - d.  A particle walks the 13 cells with d > 0 (z first,
 //    then y, then x) and its own bucket's entries after its own position t0: every pair is found
 //    from exactly one side.  The three workgroups take 5, 5 and 4 of these 14 buckets.
@@ -1026,8 +1027,8 @@ __global__ __launch_bounds__(256) void k_bp_touch_large(const lbdem_dem_params P
 // does for every candidate with gap >= 0 (demsolver.py:638-668); contacts whose partner left
 // i's probed buckets keep their slot, as in the reference.  Runs before the resolve kernels.
 // It also clears the bucket counters the substep's broad phase filled (count and current at
-// rhash[i], the hashes of the grid the table held), which k_bp_reset does for the broad phase alone:
-// nothing reads them after the probe.
+// rhash[i], the hashes of the grid the table held), as k_bp_reset does for the broad phase
+// alone: nothing reads them after the probe.
 template <int NP>
 __global__ __launch_bounds__(256) void k_dem_release(const lbdem_dem_params P, const lbdem_dem_state S,
                                                      const int32_t *hash, const int32_t *rhash, int32_t *count,

@@ -216,8 +216,8 @@ def test_dem_fine_grid_probe27_matches_oracle(periodic, table):
     """6,000 grains seeded overlapping in a 12 x 12 x 6 box (about 7 per cell), broad-phase cells
     one maximum diameter wide with the 27-cell probe, as in the unresolved bed.  The touching
     probe lists each pair from one side only: with the table of the reference's size rule (2^20
-    entries) and with 256 entries (cell coordinates kept modulo 8, 8 and 4, the least for which
-    the 27 probed hashes stay distinct) it walks the half shell of 13 cells and its own bucket;
+    entries) and with 256 entries (cell coordinates kept modulo 8, 8 and 4; the smallest table
+    the half shell is used for) it walks the half shell of 13 cells and its own bucket;
     with 32 entries (coordinates modulo 4, 4 and 2, so the 27 cells share 18 buckets) it walks
     the deduplicated buckets' entries after its own position.  The oracle runs with the same
     table (which cells alias decides which history slots the release keeps).  100, 40 and 10
